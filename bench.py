@@ -1,0 +1,202 @@
+"""SCFT benchmark: iterations/s of the AB-diblock SCFT solver on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-128]
+
+A "step" is one SCFT iteration (mean subtraction, the packed propagator march
+over Ns = 200 contour steps, Q, the trapezoid contour integral, the Anderson
+field update and the free energy) of the cylinder workload: AB diblock f=0.3,
+chiN=20, 64^3 periodic grid, box 4.2 Rg, ds=0.005, Anderson history 10, seeded
+hexagonal-star fields.  `value` is the FP32-propagator form (FP64 Q, phi, F,
+fields and least squares); the FP64 form, the 128^3 gyroid workload and the
+FP32-vs-FP64 error are reported beside it.  Fields are resident in HBM before
+the timed region; timing uses HIP events on the solver's stream.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "scft_ver1.01_amd")
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "scft_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["scft_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_params(pkg, name, prec):
+    w = pkg.fields.WORKLOADS[name]
+    n = w["n"]
+    return pkg.default_params(nx=n, ny=n, nz=n, lx=w["L"], ly=w["L"], lz=w["L"], chiN=w["chiN"],
+                              fA=w["fA"], ds=w["ds"], precision=prec, mixing=pkg.MIX_ANDERSON,
+                              anderson_m=w["anderson_m"], anderson_mix=0.1)
+
+
+def iter_bytes(N, Ns, b):
+    """Algorithmic HBM bytes of one SCFT iteration: N*[b*(6 Ns + 2 (Ns+2)) + 64]."""
+    return N * (b * (6 * Ns + 2 * (Ns + 2)) + 64)
+
+
+def run_form(pkg, name, prec, steps, warmup):
+    p = workload_params(pkg, name, prec)
+    wA, wB = pkg.fields.workload_fields(name)
+    s = pkg.Solver(p)
+    s.set_fields(wA, wB)
+    log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} warmup {warmup}")
+    s.iterate(warmup)
+    log(f"[bench] {name} timing {steps} iterations")
+    ms = s.time_iterations(steps)
+    fe = s.free_energy()
+    phiA, _, _, _ = s.get_fields()
+    return s, ms / steps, fe[0].copy(), phiA[0].copy()
+
+
+def pass_roofline(s, prec, reps=50):
+    """Mean duration of each axis pass (HIP events around `reps` back-to-back
+    launches on the solver stream) and the dominant one's bandwidth."""
+    b = 4 if prec else 8
+    out = {}
+    for axis, nm in enumerate("xyz"):
+        out[nm] = s.time_pass(axis, reps)
+    dom = max(out, key=out.get)
+    bytes_per_launch = 4 * b * s.N  # read + write of the complex field (2b bytes per point each way)
+    achieved = bytes_per_launch / (out[dom] * 1e-3) / 1e9
+    return dom, out, bytes_per_launch, achieved
+
+
+def cpu_baseline(pkg, name, budget_s=12.0, max_iters=4):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    w = pkg.fields.WORKLOADS[name]
+    n = w["n"]
+    cores = min(16, os.cpu_count() or 1)
+    orc.set_threads(cores)
+    op = orc.params(n, n, n, w["L"], w["L"], w["L"], w["chiN"], w["fA"], w["ds"])
+    kt = orc.ktable(op)
+    wA, wB = pkg.fields.workload_fields(name)
+    N = n ** 3
+    phiA = np.empty(N)
+    phiB = np.empty(N)
+    anderson = orc.Anderson(N, w["anderson_m"], 0.1, 0.10)
+    anderson.iterate(op, kt, wA, wB, phiA, phiB)  # warm-up
+    it = 0
+    t0 = time.perf_counter()
+    while it < max_iters:
+        anderson.iterate(op, kt, wA, wB, phiA, phiB)
+        it += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": it / dt, "unit": "iters/s", "cores": orc.get_threads(), "kind": "port",
+            "sample": f"{it} Anderson SCFT iterations of {name} (64^3, Ns=200, FP64) after 1 warm-up, "
+                      f"oracle C restatement with OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-128", action="store_true")
+    ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c3_fp32", "c3_fp64"],
+                    help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this solver runs one GPU per process and this build measures one GPU only; "
+              "independent SCFT instances shard across GPUs with no exchange (see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_pkg()
+    if a.profile:
+        wl, form = a.profile.split("_")
+        nm = {"c2": "c2_cylinder_64", "c3": "c3_gyroid_128"}[wl]
+        s, ms, fe, _ = run_form(pkg, nm, pkg.FP32 if form == "fp32" else pkg.FP64, a.steps, a.warmup)
+        print(json.dumps({"profile": a.profile, "ms_per_iter": ms, "iters_per_s": 1e3 / ms, "F": fe[0]}))
+        s.close()
+        return
+    name = "c2_cylinder_64"
+    s32, ms32, fe32, phi32 = run_form(pkg, name, pkg.FP32, a.steps, a.warmup)
+    s64, ms64, fe64, phi64 = run_form(pkg, name, pkg.FP64, a.steps, a.warmup)
+    Ns = s32.NsA + s32.NsB
+    dom, passes32, bpl, ach = pass_roofline(s32, 1)
+    _, passes64, bpl64, ach64 = pass_roofline(s64, 0)
+    N = s32.N
+    it32 = 1e3 / ms32
+    it64 = 1e3 / ms64
+    forms = {
+        "c2_64": {
+            "fp32_iters_per_s": it32, "fp64_iters_per_s": it64, "ratio_fp32_over_fp64": it32 / it64,
+            "fp32_iter_hbm_frac": iter_bytes(N, Ns, 4) * it32 / 1e9 / HBM_PEAK_GBPS,
+            "fp64_iter_hbm_frac": iter_bytes(N, Ns, 8) * it64 / 1e9 / HBM_PEAK_GBPS,
+            "dF_fp32_vs_fp64": abs(fe32[0] - fe64[0]),
+            "max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
+            "F_fp64": fe64[0], "iterations_compared": a.warmup + a.steps,
+            "pass_ms_fp32": passes32, "pass_ms_fp64": passes64,
+        }
+    }
+    s32.close()
+    s64.close()
+    if not a.no_128:
+        name3 = "c3_gyroid_128"
+        t32, m32, f32, p32 = run_form(pkg, name3, pkg.FP32, max(5, a.steps // 2), 3)
+        t64, m64, f64, p64 = run_form(pkg, name3, pkg.FP64, max(5, a.steps // 2), 3)
+        _, pass3_32, _, _ = pass_roofline(t32, 1, reps=20)
+        _, pass3_64, _, _ = pass_roofline(t64, 0, reps=20)
+        N3 = t32.N
+        Ns3 = t32.NsA + t32.NsB
+        forms["c3_128"] = {
+            "fp32_iters_per_s": 1e3 / m32, "fp64_iters_per_s": 1e3 / m64, "ratio_fp32_over_fp64": m64 / m32,
+            "fp32_iter_hbm_frac": iter_bytes(N3, Ns3, 4) * (1e3 / m32) / 1e9 / HBM_PEAK_GBPS,
+            "fp64_iter_hbm_frac": iter_bytes(N3, Ns3, 8) * (1e3 / m64) / 1e9 / HBM_PEAK_GBPS,
+            "dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
+            "max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
+            "pass_ms_fp32": pass3_32, "pass_ms_fp64": pass3_64,
+        }
+        t32.close()
+        t64.close()
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline (oracle, host cores)")
+        cpu = cpu_baseline(pkg, name)
+    out = {
+        "metric": "SCFT iters/s at 64^3,128^3 Ns=200 (FP32 vs FP64 form), %HBM peak, dF vs FP64",
+        "value": it32,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 propagator, f64 accumulation",
+        "data": "synthetic: seeded SplitMix64 noise + hexagonal-star field seed",
+        "config": {"workload": "c2 cylinder: AB diblock f=0.3 chiN=20, 64^3, L=4.2 Rg, ds=0.005 "
+                               "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
+                   "batch": 1},
+        "roofline": {"bound": "hbm", "kernel": f"k_line_conv {dom}-pass (fp32)", "achieved": ach,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
+                     "traffic": None, "bytes_per_launch": bpl, "ms_per_launch": passes32[dom]},
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/*
+ * scft.h -- C ABI of the MI355X SCFT solver for AB diblock-copolymer melts.
+ *
+ * The library (libscft_hip.so) replaces the reference's CUDA solver layer of
+ * ren5885a/SCFT_Ver1.01.  Every entry point returns an int status (SCFT_OK = 0)
+ * and never exits the process; scft_last_error() gives the message.  There is
+ * no global state: one context per host thread.  Host arrays are caller-owned,
+ * FP64, laid out [instance][z][y][x] with x fastest (point index
+ * r = (k*Ny + j)*Nx + i, as init_cuda.cu:425).  Device buffers are owned by
+ * the context.
+ *
+ * Correspondence with the reference (file:line):
+ *   scft_create / scft_destroy  <- init_cuda + initialize_cufft (init_cuda.h:11-13,
+ *                                  init_cuda.cu:227-546) / finalize_cufft (init_cuda.cu:551-628)
+ *   scft_set_fields             <- the phi_<n>.dat upload of initialize_cufft
+ *                                  (init_cuda.cu:435-456, 508-513)
+ *   scft_get_fields             <- host reads of the managed pha/phb/wa/wb arrays in Free
+ *                                  (cuda_scft.cu:199-243)
+ *   scft_propagate              <- sovDifFft (cuda_scft.h:1, cuda_scft.cu:435-563)
+ *   scft_concentrations         <- getConc (cuda_scft.cu:61-138)
+ *   scft_update_fields          <- phi_w launch in Free (cuda_scft.cu:176-187, cuda_aid.cu:382-416)
+ *   scft_free_energy            <- the F block of Free (cuda_scft.cu:191-223)
+ *   scft_iterate                <- the do-while loop of Free (cuda_scft.cu:167-253)
+ *   scft_time_iterations        <- fft_test's cudaEvent timing (cuda_scft.cu:382-432)
+ *   scft_contour_counts         <- NsA/dNsB of initialize_cufft (init_cuda.cu:330-331)
+ */
+#ifndef SCFT_H
+#define SCFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SCFT_OK 0
+#define SCFT_E_INVALID 1     /* bad argument or parameter */
+#define SCFT_E_UNSUPPORTED 2 /* grid length with no kernel tiling */
+#define SCFT_E_HIP 3         /* HIP runtime error (message names the call) */
+#define SCFT_E_NOMEM 4       /* device allocation failed */
+#define SCFT_E_NONFINITE 5   /* Q non-finite or <= 0, or non-finite input field */
+#define SCFT_E_IO 6          /* file missing, short or malformed */
+#define SCFT_E_STATE 7       /* call out of order (e.g. update before concentrations) */
+
+/* propagator precision */
+#define SCFT_FP64 0 /* everything FP64 (the reference's arithmetic) */
+#define SCFT_FP32 1 /* propagator, FFT and stored q in FP32; Q, phi, F, fields FP64 */
+
+/* field update */
+#define SCFT_MIX_SIMPLE 0   /* phi_w of the reference (cuda_aid.cu:382-416) */
+#define SCFT_MIX_ANDERSON 1 /* Anderson mixing, FP64 least squares (addition) */
+
+/* value of pi the reference uses for wavenumbers (struct.h:11) */
+#define SCFT_REFERENCE_PI 3.1415926535
+
+typedef struct scft_ctx scft_ctx;
+
+typedef struct {
+    int nx, ny, nz;       /* grid (Nx, Ny, Nz of ab.txt) */
+    double lx, ly, lz;    /* box lengths in Rg (lx, ly, lz of ab.txt) */
+    double chiN;          /* xN of ab.txt (hAB) */
+    double fA;            /* fa of ab.txt */
+    double ds;            /* ds0 of ab.txt */
+    int batch;            /* independent instances sharing these parameters (batch of ab.txt) */
+    int precision;        /* SCFT_FP64 or SCFT_FP32 */
+    int mixing;           /* SCFT_MIX_SIMPLE or SCFT_MIX_ANDERSON */
+    double wopt;          /* simple-mixing fraction; reference 0.05 (cuda_aid.cu:395) */
+    double wcmp;          /* incompressibility penalty; reference 0.10 (cuda_aid.cu:396) */
+    int anderson_m;       /* Anderson history length (0 = simple mixing with anderson_mix) */
+    double anderson_mix;  /* Anderson mixing fraction applied to the combined residual */
+    double pi;            /* pi used for wavenumbers; 0 selects SCFT_REFERENCE_PI */
+    int use_graph;        /* 1: iterations replay one captured HIP graph; 0: eager launches */
+} scft_params;
+
+/* Fill p with the reference's defaults (ab.txt sample sizes aside):
+   FP64, simple mixing, wopt 0.05, wcmp 0.10, pi = SCFT_REFERENCE_PI, batch 1,
+   Anderson m 10 / mix 0.1 (used only with SCFT_MIX_ANDERSON), HIP graph on. */
+void scft_default_params(scft_params *p);
+
+/* NsA and NsB exactly as the reference truncates them (init_cuda.cu:330-331). Host only. */
+int scft_contour_counts(double fA, double ds, int *NsA, int *NsB);
+
+/* Create a context on HIP device `device`: validates p, builds the wavenumber
+   and twiddle tables, allocates every device buffer.  *out is NULL on error. */
+int scft_create(const scft_params *p, int device, scft_ctx **out);
+int scft_destroy(scft_ctx *ctx);
+/* message of the last failing call on ctx (or of the last failed scft_create
+   when ctx is NULL, per host thread) */
+const char *scft_last_error(const scft_ctx *ctx);
+
+/* wA, wB: host [batch][N].  Rejects non-finite values (SCFT_E_NONFINITE). */
+int scft_set_fields(scft_ctx *ctx, const double *wA, const double *wB);
+/* any pointer may be NULL; phi is the last computed concentration */
+int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double *wA, double *wB);
+
+/* One march of block `block` (0 = A with wA and NsA steps, 1 = B with wB and
+   NsB steps) as sovDifFft: wdz = exp(-w*ds/2) from the current field (no mean
+   subtraction); sign +1 seeds slice 0 with q_init and marches up, sign -1
+   seeds slice ns and marches down.  q_init: host [batch][N], or NULL for 1.
+   q_out: host [batch][ns+1][N]. */
+int scft_propagate(scft_ctx *ctx, int block, int sign, const double *q_init, double *q_out);
+
+/* getConc: subtract the mean of wA (as getConc does, cuda_scft.cu:71), march the
+   chain, compute Q and phiA/phiB by the trapezoid rule.  Q: host [batch] or NULL. */
+int scft_concentrations(scft_ctx *ctx, double *Q);
+
+/* Field update from the last concentrations (simple mixing or Anderson per
+   params), then the free-energy terms with the updated fields. */
+int scft_update_fields(scft_ctx *ctx);
+
+/* Free-energy terms of the last update, host [batch][5]:
+   F, F_AB, F_W, F_S, inCompMax (cuda_scft.cu:199-221). */
+int scft_free_energy(scft_ctx *ctx, double *out);
+
+/* n_iter SCFT iterations, each: subtract the means of wA and wB, getConc, field
+   update, free energy (cuda_scft.cu:167-223).  If tol > 0, stop after the first
+   iteration whose inCompMax (all instances) is below tol; *done (may be NULL)
+   receives the number of iterations run.  fe (may be NULL): host [batch][5] of
+   the last iteration. */
+int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, double *fe);
+
+/* Time n_iter iterations with HIP events on the context's stream (no host
+   synchronisation inside); *ms receives the elapsed milliseconds. */
+int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
+
+/* Kernel-level timing for the roofline: runs `reps` back-to-back launches of
+   the propagator axis pass `axis` (0 x, 1 y, 2 z) on the current slice store
+   between two HIP events and returns the mean duration in ms.  Leaves the
+   solver state unchanged except the scratch field. */
+int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms);
+
+/* Problem sizes: N points, NsA, NsB, device bytes allocated. */
+int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCFT_H */

@@ -1,0 +1,253 @@
+"""Host-side Python mirror of the SCFT solver interface (ctypes over libscft_hip.so).
+
+The reference (ren5885a/SCFT_Ver1.01) drives its CUDA solver from C++ through
+these functions; the methods of `Solver` keep their roles, argument meaning
+and failure behaviour (an exception carrying the library's status instead of
+`checkCudaErrors`' exit):
+
+    Solver(params)            init_cuda + initialize_cufft      init_cuda.cu:227-546
+    Solver.close()            finalize_cufft                    init_cuda.cu:551-628
+    Solver.set_fields         phi_<n>.dat upload                init_cuda.cu:435-456, 508-513
+    Solver.propagate          sovDifFft                         cuda_scft.cu:435-563
+    Solver.concentrations     getConc                           cuda_scft.cu:61-138
+    Solver.update_fields      phi_w                             cuda_aid.cu:382-416
+    Solver.free_energy        F block of Free                   cuda_scft.cu:191-223
+    Solver.iterate            Free's do-while loop              cuda_scft.cu:167-253
+    Solver.time_iterations    fft_test                          cuda_scft.cu:382-432
+
+All compute runs in the HIP library; there is no CPU fallback.  If the
+library is missing, importing this package raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import fields  # noqa: F401  (seeded field generators)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libscft_hip.so")
+
+SCFT_OK = 0
+SCFT_E_INVALID = 1
+SCFT_E_UNSUPPORTED = 2
+SCFT_E_HIP = 3
+SCFT_E_NOMEM = 4
+SCFT_E_NONFINITE = 5
+SCFT_E_IO = 6
+SCFT_E_STATE = 7
+
+FP64 = 0
+FP32 = 1
+MIX_SIMPLE = 0
+MIX_ANDERSON = 1
+REFERENCE_PI = 3.1415926535  # struct.h:11
+
+_STATUS_NAMES = {
+    1: "SCFT_E_INVALID", 2: "SCFT_E_UNSUPPORTED", 3: "SCFT_E_HIP", 4: "SCFT_E_NOMEM",
+    5: "SCFT_E_NONFINITE", 6: "SCFT_E_IO", 7: "SCFT_E_STATE",
+}
+
+# every entry point declared in include/scft.h
+ABI_SYMBOLS = (
+    "scft_default_params", "scft_contour_counts", "scft_create", "scft_destroy",
+    "scft_last_error", "scft_set_fields", "scft_get_fields", "scft_propagate",
+    "scft_concentrations", "scft_update_fields", "scft_free_energy", "scft_iterate",
+    "scft_time_iterations", "scft_time_pass", "scft_sizes",
+)
+
+
+class ScftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """scft_params of include/scft.h"""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("chiN", ctypes.c_double), ("fA", ctypes.c_double), ("ds", ctypes.c_double),
+        ("batch", ctypes.c_int), ("precision", ctypes.c_int), ("mixing", ctypes.c_int),
+        ("wopt", ctypes.c_double), ("wcmp", ctypes.c_double),
+        ("anderson_m", ctypes.c_int), ("anderson_mix", ctypes.c_double),
+        ("pi", ctypes.c_double), ("use_graph", ctypes.c_int),
+    ]
+
+    def replace(self, **kw):
+        q = Params()
+        ctypes.pointer(q)[0] = self
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def lib():
+    """Load libscft_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python scft_ver1.01_amd/build.py` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(Params)
+        vp = ctypes.c_void_p
+        sig = {
+            "scft_default_params": (None, [P]),
+            "scft_contour_counts": (ctypes.c_int, [ctypes.c_double, ctypes.c_double,
+                                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+            "scft_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
+            "scft_destroy": (ctypes.c_int, [vp]),
+            "scft_last_error": (ctypes.c_char_p, [vp]),
+            "scft_set_fields": (ctypes.c_int, [vp, _dp, _dp]),
+            "scft_get_fields": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
+            "scft_propagate": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+            "scft_concentrations": (ctypes.c_int, [vp, _dp]),
+            "scft_update_fields": (ctypes.c_int, [vp]),
+            "scft_free_energy": (ctypes.c_int, [vp, _dp]),
+            "scft_iterate": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_int), _dp]),
+            "scft_time_iterations": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+            "scft_time_pass": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+            "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_params(**kw):
+    p = Params()
+    lib().scft_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def contour_counts(fA, ds):
+    """(NsA, NsB) truncated as init_cuda.cu:330-331."""
+    a = ctypes.c_int()
+    b = ctypes.c_int()
+    rc = lib().scft_contour_counts(fA, ds, ctypes.byref(a), ctypes.byref(b))
+    if rc:
+        raise ScftError(rc, "bad fA/ds")
+    return a.value, b.value
+
+
+def _arr(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class Solver:
+    """One solver context (batch of instances sharing params) on one HIP device."""
+
+    def __init__(self, params, device=0):
+        self.params = params
+        self._h = ctypes.c_void_p()
+        rc = lib().scft_create(ctypes.byref(params), device, ctypes.byref(self._h))
+        if rc:
+            msg = lib().scft_last_error(None).decode()
+            raise ScftError(rc, msg)
+        N = ctypes.c_long()
+        a = ctypes.c_int()
+        b = ctypes.c_int()
+        nb = ctypes.c_long()
+        lib().scft_sizes(self._h, ctypes.byref(N), ctypes.byref(a), ctypes.byref(b), ctypes.byref(nb))
+        self.N, self.NsA, self.NsB, self.device_bytes = N.value, a.value, b.value, nb.value
+        self.batch = params.batch
+
+    # -- lifecycle --
+    def close(self):
+        if self._h:
+            lib().scft_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise ScftError(rc, lib().scft_last_error(self._h).decode())
+
+    # -- fields --
+    def set_fields(self, wA, wB):
+        n = self.batch * self.N
+        wA = _arr(wA, n)
+        wB = _arr(wB, n)
+        self._check(lib().scft_set_fields(self._h, _ptr(wA), _ptr(wB)))
+
+    def get_fields(self):
+        n = self.batch * self.N
+        out = [np.empty(n) for _ in range(4)]
+        self._check(lib().scft_get_fields(self._h, *[_ptr(o) for o in out]))
+        shp = (self.batch, self.N)
+        return tuple(o.reshape(shp) for o in out)  # phiA, phiB, wA, wB
+
+    # -- solver steps --
+    def propagate(self, block, sign, q_init=None):
+        """sovDifFft: returns q[batch][ns+1][N] (FP64 copy of the device slices)."""
+        ns = self.NsA if block == 0 else self.NsB
+        qi = None if q_init is None else _arr(q_init, self.batch * self.N)
+        out = np.empty(self.batch * (ns + 1) * self.N)
+        self._check(lib().scft_propagate(self._h, block, sign, _ptr(qi), _ptr(out)))
+        return out.reshape(self.batch, ns + 1, self.N)
+
+    def concentrations(self):
+        """getConc: returns Q[batch]; phi readable with get_fields()."""
+        Q = np.empty(self.batch)
+        self._check(lib().scft_concentrations(self._h, _ptr(Q)))
+        return Q
+
+    def update_fields(self):
+        self._check(lib().scft_update_fields(self._h))
+
+    def free_energy(self):
+        out = np.empty(5 * self.batch)
+        self._check(lib().scft_free_energy(self._h, _ptr(out)))
+        return out.reshape(self.batch, 5)
+
+    def iterate(self, n, tol=0.0):
+        """Free's loop: returns (iterations run, fe[batch][5] of the last one)."""
+        done = ctypes.c_int()
+        fe = np.empty(5 * self.batch)
+        self._check(lib().scft_iterate(self._h, int(n), float(tol), ctypes.byref(done), _ptr(fe)))
+        return done.value, fe.reshape(self.batch, 5)
+
+    # -- timing --
+    def time_iterations(self, n):
+        ms = ctypes.c_float()
+        self._check(lib().scft_time_iterations(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def time_pass(self, axis, reps):
+        ms = ctypes.c_float()
+        self._check(lib().scft_time_pass(self._h, int(axis), int(reps), ctypes.byref(ms)))
+        return ms.value

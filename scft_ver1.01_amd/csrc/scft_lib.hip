@@ -1,0 +1,1271 @@
+// scft_lib.hip -- the SCFT solver context, the point-wise and reduction
+// kernels around the propagator passes, and the C ABI of include/scft.h.
+//
+// One SCFT iteration on the device (cuda_scft.cu:167-223 in the reference):
+//   k_sums      sums of wA, wB per instance (deterministic two-level reduction)
+//   k_center    w -= mean (both fields), partial sums of the new wA
+//   k_prep      wA -= mean again (getConc, cuda_scft.cu:71), wdz = exp(-w ds/2),
+//               slice 0 = 1 + i
+//   Ns x {x pass, y pass, z pass}     the packed march (see below)
+//   k_qsum      partial sums of q(Ns)
+//   k_contour   Q, then phiA/phiB by the trapezoid rule, FP64 accumulation
+//   k_mix_simple / Anderson kernels   field update + free-energy partials
+//   k_fe_final  F, F_AB, F_W, F_S, inCompMax per instance
+//
+// Packed march: the step operator S(q) = W * IFFT[K * FFT[W q]] / N has a
+// real, even k-factor, so it maps real fields to real fields and is complex
+// linear.  The chain propagator q(s) (s = 0..Ns, block A for s <= NsA) and the
+// complementary propagator qdag(s) (from s = Ns down) are therefore marched
+// together as z(t) = q(t) + i*qdag(Ns - t), t = 0..Ns, with the real and
+// imaginary parts multiplied by their own block's W.  K is separable, so the
+// 3-D convolution is three 1-D convolutions (x, y, z passes), each a batched
+// in-register FFT (pass_kernels.inc).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "scft.h"
+
+namespace scft {
+
+// ---------------------------------------------------------------------------
+// reductions (wave64 shuffles, fixed order => bitwise deterministic)
+// ---------------------------------------------------------------------------
+constexpr int RT = 256;  // threads of every point-wise / reduction kernel
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// block-wide sum of NV values per thread; result broadcast to all threads
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double *sh /* >= NV*RT/64 + NV */)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    if (l == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[i * (RT / 64) + w] = v[i];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double r = 0.0;
+        for (int k = 0; k < RT / 64; ++k) r += sh[threadIdx.x * (RT / 64) + k];
+        sh[NV * (RT / 64) + threadIdx.x] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = sh[NV * (RT / 64) + i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    v = wave_max(v);
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int k = 1; k < RT / 64; ++k) r = fmax(r, sh[k]);
+        sh[RT / 64] = r;
+    }
+    __syncthreads();
+    v = sh[RT / 64];
+    __syncthreads();
+    return v;
+}
+
+// sum of part[0..nblk) (the same tree in every block that calls it)
+__device__ __forceinline__ double finalize_sum(const double *part, int nblk, double *sh)
+{
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < nblk; k += RT) v[0] += part[k];
+    block_sums<1>(v, sh);
+    return v[0];
+}
+
+// ---------------------------------------------------------------------------
+// field kernels (FP64)
+// ---------------------------------------------------------------------------
+// part[(b*2 + f)*nblk + blk] = partial sum of field f of instance b
+__global__ void __launch_bounds__(RT) k_sums(const double *__restrict__ f0, const double *__restrict__ f1,
+                                             long N, int nblk, double *__restrict__ part)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.y;
+    double v[2] = {0.0, 0.0};
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        v[0] += f0[b * N + r];
+        if (f1) v[1] += f1[b * N + r];
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[(b * 2 + 0) * nblk + blockIdx.x] = v[0];
+        part[(b * 2 + 1) * nblk + blockIdx.x] = v[1];
+    }
+}
+
+// average_value for both fields (minus_average, cuda_aid.cu:376: data - sum/(double)N),
+// then the partial sums of the new f0 into part_out[(b*2)*nblk + blk]
+__global__ void __launch_bounds__(RT) k_center(double *__restrict__ f0, double *__restrict__ f1, long N, int nblk,
+                                               const double *__restrict__ part, double *__restrict__ part_out)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.y;
+    const double s0 = finalize_sum(part + (b * 2 + 0) * nblk, nblk, sh);
+    const double s1 = f1 ? finalize_sum(part + (b * 2 + 1) * nblk, nblk, sh) : 0.0;
+    double v[1] = {0.0};
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const double a = f0[b * N + r] - s0 / (double)N;
+        f0[b * N + r] = a;
+        v[0] += a;
+        if (f1) f1[b * N + r] = f1[b * N + r] - s1 / (double)N;
+    }
+    if (part_out) {
+        block_sums<1>(v, sh);
+        if (threadIdx.x == 0) part_out[(b * 2) * nblk + blockIdx.x] = v[0];
+    }
+}
+
+// getConc's own mean subtraction of wA (if part != null), the W = exp(-w ds/2)
+// factors of both blocks (initilize_wdz, cuda_aid.cu:254), and slice 0 = 1 + i
+template <typename T>
+__global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const double *__restrict__ wB, long N, int nblk,
+                                             const double *__restrict__ part, double ds2, T *__restrict__ wdz,
+                                             typename cplx<T>::type *__restrict__ slot0, int batch)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    using C = typename cplx<T>::type;
+    const int b = blockIdx.y;
+    const double s0 = part ? finalize_sum(part + (b * 2) * nblk, nblk, sh) : 0.0;
+    const long BN = (long)batch * N;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const long i = b * N + r;
+        double a = wA[i];
+        if (part) {
+            a = a - s0 / (double)N;
+            wA[i] = a;
+        }
+        wdz[i] = (T)exp(-a * ds2);
+        wdz[BN + i] = (T)exp(-wB[i] * ds2);
+        if (slot0) slot0[i] = C{T(1), T(1)};
+    }
+}
+
+// W of one block only (sovDifFft computes it from the field as given)
+template <typename T>
+__global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, double ds2, T *__restrict__ wdz)
+{
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
+        wdz[i] = (T)exp(-w[i] * ds2);
+}
+
+// slice <- (q_init or 1) + 0i
+template <typename T>
+__global__ void __launch_bounds__(RT) k_seed_real(typename cplx<T>::type *__restrict__ slot,
+                                                  const double *__restrict__ q, long n)
+{
+    using C = typename cplx<T>::type;
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
+        slot[i] = C{q ? (T)q[i] : T(1), T(0)};
+}
+
+// FP64 copy of the real part of a slice
+template <typename T>
+__global__ void __launch_bounds__(RT) k_take_real(const typename cplx<T>::type *__restrict__ slot,
+                                                  double *__restrict__ out, long n)
+{
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
+        out[i] = (double)slot[i].x;
+}
+
+// partial sums of Re z(Ns) = q(Ns)   (cal_ql, cuda_aid.cu:119-168)
+template <typename T>
+__global__ void __launch_bounds__(RT) k_qsum(const typename cplx<T>::type *__restrict__ slotNs, long N, int nblk,
+                                             double *__restrict__ part)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.y;
+    double v[1] = {0.0};
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT)
+        v[0] += (double)slotNs[b * N + r].x;
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[(b * 2) * nblk + blockIdx.x] = v[0];
+}
+
+// Q = sum/N and ffl = ds/Q per instance (cal_ql + host divide, cuda_scft.cu:118-126)
+__global__ void __launch_bounds__(RT) k_q_final(const double *__restrict__ part, int nblk, long N, double ds,
+                                                double *__restrict__ Qout, double *__restrict__ ffl)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.x;
+    const double Q = finalize_sum(part + (b * 2) * nblk, nblk, sh) / N;
+    if (threadIdx.x == 0) {
+        Qout[b] = Q;
+        ffl[b] = ds / Q;
+    }
+}
+
+// trapezoid weight of contour point s for block A and block B (w_to_phi_go,
+// cuda_aid.cu:87-107): A covers s = 0..NsA, B covers s = NsA..Ns, ends 1/2
+__device__ __forceinline__ void trap_w(int s, int NsA, int Ns, double &wa, double &wb)
+{
+    wa = (s < NsA) ? ((s == 0) ? 0.5 : 1.0) : ((s == NsA) ? 0.5 : 0.0);
+    wb = (s > NsA) ? ((s == Ns) ? 0.5 : 1.0) : ((s == NsA) ? 0.5 : 0.0);
+}
+
+// phi by the trapezoid rule with FP64 accumulation (w_to_phi_go, cuda_aid.cu:61-118).
+// Contour point s pairs q(s) = Re z(s) with qdag(s) = Im z(Ns - s); slices s and
+// Ns - s are loaded once for both products.  One point per thread, 4 slice
+// pairs in flight.
+template <typename T>
+__global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__restrict__ slots, long N, int batch,
+                                                int NsA, int Ns, const double *__restrict__ ffl,
+                                                double *__restrict__ phiA, double *__restrict__ phiB)
+{
+    using C = typename cplx<T>::type;
+    const int b = blockIdx.y;
+    const long r = (long)blockIdx.x * RT + threadIdx.x;
+    if (r >= N) return;
+    const long BN = (long)batch * N;
+    const C *zp = slots + b * N + r;
+    double accA = 0.0, accB = 0.0;
+    const int half = Ns / 2;  // pairs s = 0..half (s == Ns - s only when Ns is even and s == half)
+    int s = 0;
+    for (; s + 3 <= half; s += 4) {
+        C zs[4], zu[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            zs[k] = zp[(long)(s + k) * BN];
+            zu[k] = zp[(long)(Ns - s - k) * BN];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s1 = s + k, u1 = Ns - s - k;
+            double wa, wb;
+            const double p1 = (double)zs[k].x * (double)zu[k].y;  // q(s1) qdag(s1)
+            trap_w(s1, NsA, Ns, wa, wb);
+            accA += wa * p1;
+            accB += wb * p1;
+            if (u1 != s1) {
+                const double p2 = (double)zu[k].x * (double)zs[k].y;  // q(u1) qdag(u1)
+                trap_w(u1, NsA, Ns, wa, wb);
+                accA += wa * p2;
+                accB += wb * p2;
+            }
+        }
+    }
+    for (; s <= half; ++s) {
+        const int u1 = Ns - s;
+        const C a = zp[(long)s * BN], c = zp[(long)u1 * BN];
+        double wa, wb;
+        const double p1 = (double)a.x * (double)c.y;
+        trap_w(s, NsA, Ns, wa, wb);
+        accA += wa * p1;
+        accB += wb * p1;
+        if (u1 != s) {
+            const double p2 = (double)c.x * (double)a.y;
+            trap_w(u1, NsA, Ns, wa, wb);
+            accA += wa * p2;
+            accB += wb * p2;
+        }
+    }
+    const double f = ffl[b];
+    phiA[b * N + r] = accA * f;
+    phiB[b * N + r] = accB * f;
+}
+
+// free-energy partials with the updated fields: fpart[(b*3 + j)*nblk + blk],
+// j = 0: sum chiN*phiA*phiB, 1: -sum(wA phiA + wB phiB), 2: max|1 - phiA - phiB|
+__device__ __forceinline__ void fe_partials(double fab, double fw, double inc, double *sh, double *fpart, int b,
+                                            int nblk)
+{
+    double v[2] = {fab, fw};
+    block_sums<2>(v, sh);
+    const double m = block_max(inc, sh);
+    if (threadIdx.x == 0) {
+        fpart[(b * 3 + 0) * nblk + blockIdx.x] = v[0];
+        fpart[(b * 3 + 1) * nblk + blockIdx.x] = v[1];
+        fpart[(b * 3 + 2) * nblk + blockIdx.x] = m;
+    }
+}
+
+// phi_w (cuda_aid.cu:382-416) + F partials (cuda_scft.cu:202-210)
+__global__ void __launch_bounds__(RT) k_mix_simple(double *__restrict__ wA, double *__restrict__ wB,
+                                                   const double *__restrict__ phiA, const double *__restrict__ phiB,
+                                                   long N, int nblk, double hAB, double wopt, double wcmp,
+                                                   double *__restrict__ fpart)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.y;
+    double fab = 0.0, fw = 0.0, inc = 0.0;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const long i = b * N + r;
+        const double pa = phiA[i], pb = phiB[i];
+        double a = wA[i], c = wB[i];
+        const double eta = (a + c - hAB) / 2;
+        const double psum = 1.0 - pa - pb;
+        double waDiff = hAB * pb + eta - a;
+        double wbDiff = hAB * pa + eta - c;
+        waDiff -= wcmp * psum;
+        wbDiff -= wcmp * psum;
+        a += wopt * waDiff;
+        c += wopt * wbDiff;
+        wA[i] = a;
+        wB[i] = c;
+        inc = fmax(inc, fabs(1 - pa - pb));
+        fab = fab + hAB * pa * pb;
+        fw = fw - (a * pa + c * pb);
+    }
+    fe_partials(fab, fw, inc, sh, fpart, b, nblk);
+}
+
+// fe[b*5 + .] = F, F_AB, F_W, F_S, inCompMax  (cuda_scft.cu:212-218)
+__global__ void __launch_bounds__(RT) k_fe_final(const double *__restrict__ fpart, int nblk, const double *__restrict__ Q,
+                                                 long N, double *__restrict__ fe, long *and_cnt)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.x;
+    const double fab = finalize_sum(fpart + (b * 3 + 0) * nblk, nblk, sh);
+    const double fw = finalize_sum(fpart + (b * 3 + 1) * nblk, nblk, sh);
+    double inc = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += RT) inc = fmax(inc, fpart[(b * 3 + 2) * nblk + k]);
+    inc = block_max(inc, sh);
+    if (threadIdx.x == 0) {
+        const double freeAB = fab / N;
+        const double freeW = fw / N;
+        const double freeS = -log(Q[b]);
+        fe[b * 5 + 0] = freeAB + freeW + freeS;
+        fe[b * 5 + 1] = freeAB;
+        fe[b * 5 + 2] = freeW;
+        fe[b * 5 + 3] = freeS;
+        fe[b * 5 + 4] = inc;
+        if (and_cnt && b == 0) *and_cnt += 1;  // one Anderson update done (all instances in lockstep)
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Anderson mixing (no reference counterpart).  History ring of m+1 slots of
+// (w, d) per instance, FP64.  Residual (the direction simple mixing steps in):
+//   eta = (wA+wB-chiN)/2, psi = 1-phiA-phiB,
+//   dA = chiN phiB + eta - wA - wcmp psi,  dB = chiN phiA + eta - wB - wcmp psi.
+// Gram matrix G[i][j] = <d_i, d_j> over both fields; for the newest slot c and
+// history slots s_1..s_m':  U_ij = G_cc - G_cs_i - G_cs_j + G_s_i s_j,
+// V_i = G_cc - G_cs_i; U c = V by Gaussian elimination with partial pivoting;
+// w+ = w_c + sum c_i (w_si - w_c) + mix (d_c + sum c_i (d_si - d_c)).
+// ---------------------------------------------------------------------------
+constexpr int AND_MAX = 32;
+
+// Ring bookkeeping from the device-side update counter *cnt (updates done so
+// far): newest slot s[0] = cnt mod (m+1), history s[k] = (cnt - k) mod (m+1),
+// mp = min(cnt, m) history entries.  Reading the counter on the device keeps
+// every kernel argument fixed from one iteration to the next.
+struct AndRing {
+    int mp;
+    int s[AND_MAX + 1];
+};
+__device__ __forceinline__ AndRing and_ring(const long *cnt, int m)
+{
+    AndRing r;
+    const long c = *cnt;
+    const int m1 = m + 1;
+    r.mp = (int)(c < m ? c : m);
+    const int cur = (int)(c % m1);
+    for (int k = 0; k <= AND_MAX; ++k) r.s[k] = ((cur - k) % m1 + m1) % m1;
+    return r;
+}
+
+// residual into the newest ring slot; partial inner products
+// part[(b*(MAXM+2) + j)*nblk + blk]: j < mp: <d, d_{s[j+1]}>, j = MAXM: <d,d>, j = MAXM+1: <w,w>
+template <int MAXM>
+__global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA, const double *__restrict__ wB,
+                                                  const double *__restrict__ phiA, const double *__restrict__ phiB,
+                                                  long N, int batch, int nblk, double hAB, double wcmp,
+                                                  double *__restrict__ wh, double *__restrict__ dh, const long *cnt,
+                                                  int m, double *__restrict__ part)
+{
+    __shared__ double sh[(MAXM + 2) * RT / 64 + MAXM + 2 + 2];
+    const int b = blockIdx.y;
+    const AndRing rg = and_ring(cnt, m);
+    const long slot = 2 * (long)batch * N;  // one ring slot: [batch][2][N]
+    double acc[MAXM + 2];
+#pragma unroll
+    for (int j = 0; j < MAXM + 2; ++j) acc[j] = 0.0;
+    double *wc = wh + rg.s[0] * slot + b * 2 * N;
+    double *dc = dh + rg.s[0] * slot + b * 2 * N;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const long i = b * N + r;
+        const double a = wA[i], c = wB[i], pa = phiA[i], pb = phiB[i];
+        const double eta = (a + c - hAB) / 2;
+        const double psum = 1.0 - pa - pb;
+        const double dA = hAB * pb + eta - a - wcmp * psum;
+        const double dB = hAB * pa + eta - c - wcmp * psum;
+        wc[r] = a;
+        wc[N + r] = c;
+        dc[r] = dA;
+        dc[N + r] = dB;
+#pragma unroll
+        for (int j = 0; j < MAXM; ++j) {
+            if (j < rg.mp) {
+                const double *dj = dh + rg.s[j + 1] * slot + b * 2 * N;
+                acc[j] += dA * dj[r] + dB * dj[N + r];
+            }
+        }
+        acc[MAXM] += dA * dA + dB * dB;
+        acc[MAXM + 1] += a * a + c * c;
+    }
+    block_sums<MAXM + 2>(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < MAXM + 2; ++j) part[(b * (MAXM + 2) + j) * nblk + blockIdx.x] = acc[j];
+}
+
+// one workgroup per instance: finish the new Gram row, solve, write coefficients
+__global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ part, int nblk, int maxm, int m,
+                                                  const long *cnt, double *__restrict__ G, double *__restrict__ coef,
+                                                  double *__restrict__ err)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    __shared__ double row[AND_MAX + 2];
+    __shared__ double U[AND_MAX * AND_MAX];
+    __shared__ double V[AND_MAX];
+    const int b = blockIdx.x;
+    const AndRing rg = and_ring(cnt, m);
+    const int mp = rg.mp, m1 = m + 1;
+    for (int j = 0; j < maxm + 2; ++j) {
+        if (j >= mp && j < maxm) continue;
+        const double v = finalize_sum(part + (b * (maxm + 2) + j) * nblk, nblk, sh);
+        if (threadIdx.x == 0) row[j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double *g = G + (long)b * m1 * m1;
+    const int c = rg.s[0];
+    for (int j = 0; j < mp; ++j) {
+        g[c * m1 + rg.s[j + 1]] = row[j];
+        g[rg.s[j + 1] * m1 + c] = row[j];
+    }
+    g[c * m1 + c] = row[maxm];
+    err[b] = row[maxm + 1] > 0.0 ? sqrt(row[maxm] / row[maxm + 1]) : sqrt(row[maxm]);
+    const double gcc = g[c * m1 + c];
+    for (int i = 0; i < mp; ++i) {
+        const int si = rg.s[i + 1];
+        V[i] = gcc - g[c * m1 + si];
+        for (int j = 0; j < mp; ++j) {
+            const int sj = rg.s[j + 1];
+            U[i * mp + j] = gcc - g[c * m1 + si] - g[c * m1 + sj] + g[si * m1 + sj];
+        }
+    }
+    // Gaussian elimination with partial pivoting (same steps as the oracle)
+    bool ok = true;
+    for (int k = 0; k < mp && ok; ++k) {
+        int piv = k;
+        double best = fabs(U[k * mp + k]);
+        for (int i = k + 1; i < mp; ++i)
+            if (fabs(U[i * mp + k]) > best) {
+                best = fabs(U[i * mp + k]);
+                piv = i;
+            }
+        if (best == 0.0) {
+            ok = false;
+            break;
+        }
+        if (piv != k) {
+            for (int j = 0; j < mp; ++j) {
+                const double t = U[k * mp + j];
+                U[k * mp + j] = U[piv * mp + j];
+                U[piv * mp + j] = t;
+            }
+            const double t = V[k];
+            V[k] = V[piv];
+            V[piv] = t;
+        }
+        for (int i = k + 1; i < mp; ++i) {
+            const double f = U[i * mp + k] / U[k * mp + k];
+            for (int j = k; j < mp; ++j) U[i * mp + j] -= f * U[k * mp + j];
+            V[i] -= f * V[k];
+        }
+    }
+    if (ok)
+        for (int k = mp - 1; k >= 0; --k) {
+            double s = V[k];
+            for (int j = k + 1; j < mp; ++j) s -= U[k * mp + j] * V[j];
+            V[k] = s / U[k * mp + k];
+        }
+    for (int i = 0; i < AND_MAX; ++i) coef[b * AND_MAX + i] = (ok && i < mp) ? V[i] : 0.0;
+}
+
+// w+ from the ring, then F partials with the updated fields
+__global__ void __launch_bounds__(RT) k_and_update(double *__restrict__ wA, double *__restrict__ wB,
+                                                   const double *__restrict__ phiA, const double *__restrict__ phiB,
+                                                   long N, int batch, int nblk, int m, const long *cnt, double mix,
+                                                   double hAB, const double *__restrict__ wh,
+                                                   const double *__restrict__ dh, const double *__restrict__ coef,
+                                                   double *__restrict__ fpart)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    __shared__ double cs[AND_MAX];
+    const int b = blockIdx.y;
+    const AndRing rg = and_ring(cnt, m);
+    const int mp = rg.mp;
+    if (threadIdx.x < AND_MAX) cs[threadIdx.x] = coef[b * AND_MAX + threadIdx.x];
+    __syncthreads();
+    const long slot = 2 * (long)batch * N;
+    const double *w0 = wh + rg.s[0] * slot + b * 2 * N;
+    const double *d0 = dh + rg.s[0] * slot + b * 2 * N;
+    double fab = 0.0, fw = 0.0, inc = 0.0;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const double wa0 = w0[r], wb0 = w0[N + r], da0 = d0[r], db0 = d0[N + r];
+        double wa = wa0, wb = wb0, da = da0, db = db0;
+        for (int k = 0; k < mp; ++k) {
+            const double *wk = wh + rg.s[k + 1] * slot + b * 2 * N;
+            const double *dk = dh + rg.s[k + 1] * slot + b * 2 * N;
+            const double c = cs[k];
+            wa += c * (wk[r] - wa0);
+            wb += c * (wk[N + r] - wb0);
+            da += c * (dk[r] - da0);
+            db += c * (dk[N + r] - db0);
+        }
+        const double a = wa + mix * da, cB = wb + mix * db;
+        const long i = b * N + r;
+        wA[i] = a;
+        wB[i] = cB;
+        const double pa = phiA[i], pb = phiB[i];
+        inc = fmax(inc, fabs(1 - pa - pb));
+        fab = fab + hAB * pa * pb;
+        fw = fw - (a * pa + cB * pb);
+    }
+    fe_partials(fab, fw, inc, sh, fpart, b, nblk);
+}
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+struct SolverBase {
+    virtual ~SolverBase() = default;
+};
+
+}  // namespace scft
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace scft;
+
+static thread_local std::string g_create_error;
+
+struct scft_ctx {
+    scft_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int NsA = 0, NsB = 0, Ns = 0;
+    long N = 0;
+    int batch = 1;
+    int fp32 = 0;
+    int n[3] = {1, 1, 1};
+    int active[3] = {0, 0, 0};
+    int first_axis = -1, last_axis = -1;
+    int nblk = 1;
+    size_t csize = 16;  // bytes per complex element
+    size_t tsize = 8;   // bytes per real element
+    // device buffers
+    double *wA = nullptr, *wB = nullptr, *phiA = nullptr, *phiB = nullptr;
+    void *wdz = nullptr;    // T [2][batch][N]
+    void *slots = nullptr;  // C [Ns+1][batch][N]
+    void *work = nullptr;   // C [batch][N]
+    double *part = nullptr, *part2 = nullptr, *fpart = nullptr;
+    double *Q = nullptr, *fe = nullptr, *ffl = nullptr;
+    void *tw[3] = {nullptr, nullptr, nullptr};
+    void *kf[3] = {nullptr, nullptr, nullptr};
+    // Anderson
+    double *wh = nullptr, *dh = nullptr, *G = nullptr, *coef = nullptr, *aerr = nullptr, *apart = nullptr;
+    long *and_cnt = nullptr;  // device: Anderson updates done since the last set_fields
+    int and_maxm = 0;         // accumulator bucket (8, 16 or 32) >= anderson_m
+    // one SCFT iteration captured as a HIP graph (built on first use)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int have_conc = 0;
+    int have_fe = 0;
+    long device_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+static int set_err(scft_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPX(ctx, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_err((ctx), SCFT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+static int dalloc(scft_ctx *c, void **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return set_err(c, SCFT_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " +
+                                            hipGetErrorString(e));
+    }
+    c->allocs.push_back(*ptr);
+    c->device_bytes += (long)bytes;
+    return SCFT_OK;
+}
+
+extern "C" void scft_default_params(scft_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 32;
+    p->lx = p->ly = p->lz = 4.0;
+    p->chiN = 15.0;
+    p->fA = 0.5;
+    p->ds = 0.01;
+    p->batch = 1;
+    p->precision = SCFT_FP64;
+    p->mixing = SCFT_MIX_SIMPLE;
+    p->wopt = 0.050;
+    p->wcmp = 0.10;
+    p->anderson_m = 10;
+    p->anderson_mix = 0.1;
+    p->pi = SCFT_REFERENCE_PI;
+    p->use_graph = 1;
+}
+
+// init_cuda.cu:326-331
+extern "C" int scft_contour_counts(double fA, double ds, int *NsA, int *NsB)
+{
+    if (!NsA || !NsB || !(ds > 0.0) || !std::isfinite(fA) || !std::isfinite(ds)) return SCFT_E_INVALID;
+    const double dfB = 1 - fA;
+    *NsA = ((int)(fA / ds + 1.0e-8));
+    *NsB = ((int)(dfB / ds + 1.0e-8));
+    return SCFT_OK;
+}
+
+extern "C" const char *scft_last_error(const scft_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+static void free_all(scft_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    for (void *ptr : c->allocs) (void)hipFree(ptr);
+    c->allocs.clear();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->ev0 = c->ev1 = nullptr;
+    c->stream = nullptr;
+}
+
+extern "C" int scft_destroy(scft_ctx *ctx)
+{
+    if (!ctx) return SCFT_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return SCFT_OK;
+}
+
+// k^2 of one axis exactly as init_cuda.cu:408-418 (dx = l/(double)n)
+static void ksq_axis(int n, double l, double pi, std::vector<double> &ksq)
+{
+    const double dx = l / (double)n;
+    ksq.assign(n, 0.0);
+    for (int i = 0; i <= n / 2 - 1; i++) ksq[i] = 2 * pi * i * 1.0 / n / dx;
+    for (int i = n / 2; i < n; i++) ksq[i] = 2 * pi * (i - n) * 1.0 / dx / n;
+    for (int i = 0; i < n; i++) ksq[i] *= ksq[i];
+}
+
+static int upload_tables(scft_ctx *c)
+{
+    const double pi = c->p.pi;
+    const double ls[3] = {c->p.lx, c->p.ly, c->p.lz};
+    std::vector<double> k2[3];
+    for (int a = 0; a < 3; ++a) ksq_axis(c->n[a], ls[a], pi, k2[a]);
+    // axes of length 1 are not transformed; their single factor exp(-ds k^2[0])
+    // (nonzero: the reference's table gives k = -2 pi/l there) folds into the first pass
+    double fold = 1.0;
+    for (int a = 0; a < 3; ++a)
+        if (!c->active[a]) fold *= std::exp(-c->p.ds * k2[a][0]);
+    for (int a = 0; a < 3; ++a) {
+        if (!c->active[a]) continue;
+        const int n = c->n[a];
+        std::vector<double> kf(n), twr(n), twi(n);
+        for (int k = 0; k < n; ++k) {
+            kf[k] = std::exp(-c->p.ds * k2[a][k]) / n;
+            if (a == c->first_axis) kf[k] *= fold;
+            // exact-angle twiddles, long double then rounded
+            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+            twr[k] = (double)cosl(ang);
+            twi[k] = (double)-sinl(ang);
+        }
+        int rc;
+        if (c->fp32) {
+            std::vector<float> kff(n), tw(2 * n);
+            for (int k = 0; k < n; ++k) {
+                kff[k] = (float)kf[k];
+                tw[2 * k] = (float)twr[k];
+                tw[2 * k + 1] = (float)twi[k];
+            }
+            if ((rc = dalloc(c, &c->kf[a], sizeof(float) * n))) return rc;
+            if ((rc = dalloc(c, &c->tw[a], sizeof(float) * 2 * n))) return rc;
+            HIPX(c, hipMemcpy(c->kf[a], kff.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> tw(2 * n);
+            for (int k = 0; k < n; ++k) {
+                tw[2 * k] = twr[k];
+                tw[2 * k + 1] = twi[k];
+            }
+            if ((rc = dalloc(c, &c->kf[a], sizeof(double) * n))) return rc;
+            if ((rc = dalloc(c, &c->tw[a], sizeof(double) * 2 * n))) return rc;
+            HIPX(c, hipMemcpy(c->kf[a], kf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+        }
+    }
+    return SCFT_OK;
+}
+
+extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
+{
+    if (!out) return set_err(nullptr, SCFT_E_INVALID, "scft_create: out is NULL");
+    *out = nullptr;
+    if (!pin) return set_err(nullptr, SCFT_E_INVALID, "scft_create: params is NULL");
+    scft_params p = *pin;
+    if (p.pi == 0.0) p.pi = SCFT_REFERENCE_PI;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1) return set_err(nullptr, SCFT_E_INVALID, "grid sizes must be >= 1");
+    if (!(p.lx > 0 && p.ly > 0 && p.lz > 0) || !std::isfinite(p.lx) || !std::isfinite(p.ly) || !std::isfinite(p.lz))
+        return set_err(nullptr, SCFT_E_INVALID, "box lengths must be finite and > 0");
+    if (!(p.ds > 0.0) || !(p.ds < 0.5)) return set_err(nullptr, SCFT_E_INVALID, "ds must be in (0, 0.5)");
+    if (!(p.fA > 0.0 && p.fA < 1.0)) return set_err(nullptr, SCFT_E_INVALID, "fA must be in (0, 1)");
+    if (!std::isfinite(p.chiN)) return set_err(nullptr, SCFT_E_INVALID, "chiN must be finite");
+    if (p.batch < 1) return set_err(nullptr, SCFT_E_INVALID, "batch must be >= 1");
+    if (p.precision != SCFT_FP64 && p.precision != SCFT_FP32)
+        return set_err(nullptr, SCFT_E_INVALID, "precision must be SCFT_FP64 or SCFT_FP32");
+    if (p.mixing != SCFT_MIX_SIMPLE && p.mixing != SCFT_MIX_ANDERSON)
+        return set_err(nullptr, SCFT_E_INVALID, "mixing must be SCFT_MIX_SIMPLE or SCFT_MIX_ANDERSON");
+    if (p.mixing == SCFT_MIX_ANDERSON && (p.anderson_m < 0 || p.anderson_m > AND_MAX))
+        return set_err(nullptr, SCFT_E_INVALID, "anderson_m must be in [0, 32]");
+    int NsA, NsB;
+    scft_contour_counts(p.fA, p.ds, &NsA, &NsB);
+    if (NsA < 1 || NsB < 1)
+        return set_err(nullptr, SCFT_E_INVALID, "both blocks need at least one contour step (fA/ds, (1-fA)/ds >= 1)");
+    const int n[3] = {p.nx, p.ny, p.nz};
+    for (int a = 0; a < 3; ++a)
+        if (n[a] > 1 && !pass_supported(n[a]))
+            return set_err(nullptr, SCFT_E_UNSUPPORTED,
+                           "grid length " + std::to_string(n[a]) + " has no kernel tiling (supported: 1..12 "
+                           "(2,3,5-smooth), 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 128, 144, 160, 192, 256)");
+    if (p.nx == 1 && p.ny == 1 && p.nz == 1) return set_err(nullptr, SCFT_E_INVALID, "grid must have a length > 1");
+
+    std::unique_ptr<scft_ctx> c(new scft_ctx());
+    c->p = p;
+    c->device = device;
+    c->NsA = NsA;
+    c->NsB = NsB;
+    c->Ns = NsA + NsB;
+    c->N = (long)p.nx * p.ny * p.nz;
+    c->batch = p.batch;
+    c->fp32 = p.precision == SCFT_FP32;
+    c->csize = c->fp32 ? 8 : 16;
+    c->tsize = c->fp32 ? 4 : 8;
+    for (int a = 0; a < 3; ++a) {
+        c->n[a] = n[a];
+        c->active[a] = n[a] > 1;
+        if (c->active[a]) {
+            if (c->first_axis < 0) c->first_axis = a;
+            c->last_axis = a;
+        }
+    }
+    // reduction blocks per instance: ~2 points per thread, at most 2048 partials
+    c->nblk = (int)std::min<long>(2048, std::max<long>(1, (c->N + RT * 2 - 1) / (RT * 2)));
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(nullptr, SCFT_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_err(nullptr, SCFT_E_INVALID, "device index out of range");
+    {
+        hipError_t e = hipSetDevice(device);
+        if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        e = hipEventCreate(&c->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+        if (e != hipSuccess) {
+            free_all(c.get());
+            return set_err(nullptr, SCFT_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    const size_t BN = (size_t)c->batch * c->N;
+    int rc = SCFT_OK;
+    auto A = [&](void **ptr, size_t bytes) {
+        if (rc == SCFT_OK) rc = dalloc(c.get(), ptr, bytes);
+    };
+    A((void **)&c->wA, 8 * BN);
+    A((void **)&c->wB, 8 * BN);
+    A((void **)&c->phiA, 8 * BN);
+    A((void **)&c->phiB, 8 * BN);
+    A(&c->wdz, 2 * c->tsize * BN);
+    A(&c->slots, (size_t)(c->Ns + 1) * c->csize * BN);
+    A(&c->work, c->csize * BN);
+    A((void **)&c->part, sizeof(double) * 2 * c->batch * c->nblk);
+    A((void **)&c->part2, sizeof(double) * 2 * c->batch * c->nblk);
+    A((void **)&c->fpart, sizeof(double) * 3 * c->batch * c->nblk);
+    A((void **)&c->Q, sizeof(double) * c->batch);
+    A((void **)&c->fe, sizeof(double) * 5 * c->batch);
+    A((void **)&c->ffl, sizeof(double) * c->batch);
+    if (p.mixing == SCFT_MIX_ANDERSON) {
+        const int m1 = p.anderson_m + 1;
+        c->and_maxm = p.anderson_m <= 8 ? 8 : (p.anderson_m <= 16 ? 16 : 32);
+        A((void **)&c->and_cnt, sizeof(long));
+        A((void **)&c->wh, sizeof(double) * m1 * 2 * BN);
+        A((void **)&c->dh, sizeof(double) * m1 * 2 * BN);
+        A((void **)&c->G, sizeof(double) * c->batch * m1 * m1);
+        A((void **)&c->coef, sizeof(double) * c->batch * AND_MAX);
+        A((void **)&c->aerr, sizeof(double) * c->batch);
+        A((void **)&c->apart, sizeof(double) * c->batch * (c->and_maxm + 2) * c->nblk);
+    }
+    if (rc != SCFT_OK) {
+        std::string m = c->err;
+        free_all(c.get());
+        return set_err(nullptr, rc, m);
+    }
+    if ((rc = upload_tables(c.get())) != SCFT_OK) {
+        std::string m = c->err;
+        free_all(c.get());
+        return set_err(nullptr, rc, m);
+    }
+    {
+        hipError_t e = hipMemsetAsync(c->wA, 0, 8 * BN, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->wB, 0, 8 * BN, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->phiA, 0, 8 * BN, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->phiB, 0, 8 * BN, c->stream);
+        if (e == hipSuccess && c->and_cnt) e = hipMemsetAsync(c->and_cnt, 0, sizeof(long), c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            free_all(c.get());
+            return set_err(nullptr, SCFT_E_HIP, std::string("initial clear: ") + hipGetErrorString(e));
+        }
+    }
+    *out = c.release();
+    return SCFT_OK;
+}
+
+extern "C" int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (N) *N = ctx->N;
+    if (NsA) *NsA = ctx->NsA;
+    if (NsB) *NsB = ctx->NsB;
+    if (device_bytes) *device_bytes = ctx->device_bytes;
+    return SCFT_OK;
+}
+
+extern "C" int scft_set_fields(scft_ctx *ctx, const double *wA, const double *wB)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (!wA || !wB) return set_err(ctx, SCFT_E_INVALID, "scft_set_fields: NULL field");
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    for (size_t i = 0; i < BN; ++i)
+        if (!std::isfinite(wA[i]) || !std::isfinite(wB[i]))
+            return set_err(ctx, SCFT_E_NONFINITE, "scft_set_fields: non-finite value at index " + std::to_string(i));
+    HIPX(ctx, hipSetDevice(ctx->device));
+    HIPX(ctx, hipMemcpyAsync(ctx->wA, wA, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(ctx, hipMemcpyAsync(ctx->wB, wB, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_conc = 0;
+    ctx->have_fe = 0;
+    if (ctx->and_cnt) {  // a new field restarts the Anderson history
+        const long zero = 0;
+        HIPX(ctx, hipMemcpy(ctx->and_cnt, &zero, sizeof(long), hipMemcpyHostToDevice));
+    }
+    return SCFT_OK;
+}
+
+extern "C" int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double *wA, double *wB)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    if (phiA) HIPX(ctx, hipMemcpyAsync(phiA, ctx->phiA, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (phiB) HIPX(ctx, hipMemcpyAsync(phiB, ctx->phiB, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (wA) HIPX(ctx, hipMemcpyAsync(wA, ctx->wA, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (wB) HIPX(ctx, hipMemcpyAsync(wB, ctx->wB, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    return SCFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// enqueue helpers (no host synchronisation)
+// ---------------------------------------------------------------------------
+static unsigned pt_grid(long n) { return (unsigned)std::min<long>(4096, std::max<long>(1, (n + RT - 1) / RT)); }
+
+template <typename T>
+static hipError_t enqueue_march(scft_ctx *c, int nsteps, const T *wre_t, const T *wim_t, bool packed)
+{
+    using C = typename cplx<T>::type;
+    const size_t BN = (size_t)c->batch * c->N;
+    C *slots = (C *)c->slots;
+    C *work = (C *)c->work;
+    const T *wdzA = (const T *)c->wdz;
+    const T *wdzB = wdzA + BN;
+    for (int t = 1; t <= nsteps; ++t) {
+        const T *wre, *wim;
+        if (packed) {
+            wre = (t <= c->NsA) ? wdzA : wdzB;  // q(t): block A for t <= NsA
+            wim = (t <= c->NsB) ? wdzB : wdzA;  // qdag(Ns-t): block B for t <= NsB
+        } else {
+            wre = wre_t;
+            wim = wim_t;
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (!c->active[a]) continue;
+            PassArgs<T> pa;
+            pa.src = (a == c->first_axis) ? slots + (size_t)(t - 1) * BN : work;
+            pa.dst = (a == c->last_axis) ? slots + (size_t)t * BN : work;
+            pa.wre = wre;
+            pa.wim = wim;
+            pa.tw = (const C *)c->tw[a];
+            pa.kf = (const T *)c->kf[a];
+            pa.nlines = (long)c->batch * (c->N / c->n[a]);
+            pa.N = c->N;
+            pa.nx = c->n[0];
+            pa.ny = c->n[1];
+            pa.nz = c->n[2];
+            pa.pre = (a == c->first_axis);
+            pa.post = (a == c->last_axis);
+            hipError_t e = launch_pass<T>(a, c->n[a], pa, c->stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// getConc on the device; mean_part: partial sums of wA to subtract first (or null)
+template <typename T>
+static hipError_t enqueue_conc(scft_ctx *c, const double *mean_part)
+{
+    using C = typename cplx<T>::type;
+    const size_t BN = (size_t)c->batch * c->N;
+    const dim3 g(c->nblk, c->batch);
+    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->N, c->nblk, mean_part,
+                       c->p.ds / 2, (T *)c->wdz, (C *)c->slots, c->batch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = enqueue_march<T>(c, c->Ns, nullptr, nullptr, true)) != hipSuccess) return e;
+    const C *slotNs = (const C *)c->slots + (size_t)c->Ns * BN;
+    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, slotNs, c->N, c->nblk, c->part2);
+    hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
+                       c->ffl);
+    const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+    hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch, c->NsA, c->Ns,
+                       c->ffl, c->phiA, c->phiB);
+    return hipGetLastError();
+}
+
+static hipError_t enqueue_conc_any(scft_ctx *c, const double *mean_part)
+{
+    return c->fp32 ? enqueue_conc<float>(c, mean_part) : enqueue_conc<double>(c, mean_part);
+}
+
+static hipError_t enqueue_update(scft_ctx *c)
+{
+    const dim3 g(c->nblk, c->batch);
+    if (c->p.mixing == SCFT_MIX_SIMPLE) {
+        hipLaunchKernelGGL(k_mix_simple, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->nblk,
+                           c->p.chiN, c->p.wopt, c->p.wcmp, c->fpart);
+    } else {
+        const int m = c->p.anderson_m;
+        if (c->and_maxm == 8)
+            hipLaunchKernelGGL(k_and_resid<8>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
+                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+        else if (c->and_maxm == 16)
+            hipLaunchKernelGGL(k_and_resid<16>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
+                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+        else
+            hipLaunchKernelGGL(k_and_resid<32>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
+                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+        hipLaunchKernelGGL(k_and_solve, dim3(c->batch), dim3(RT), 0, c->stream, c->apart, c->nblk, c->and_maxm, m,
+                           c->and_cnt, c->G, c->coef, c->aerr);
+        hipLaunchKernelGGL(k_and_update, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch,
+                           c->nblk, m, c->and_cnt, c->p.anderson_mix, c->p.chiN, c->wh, c->dh, c->coef, c->fpart);
+    }
+    hipLaunchKernelGGL(k_fe_final, dim3(c->batch), dim3(RT), 0, c->stream, c->fpart, c->nblk, c->Q, c->N, c->fe,
+                       c->p.mixing == SCFT_MIX_ANDERSON ? c->and_cnt : nullptr);
+    return hipGetLastError();
+}
+
+// one full iteration of Free's loop
+static hipError_t enqueue_iteration(scft_ctx *c)
+{
+    const dim3 g(c->nblk, c->batch);
+    hipLaunchKernelGGL(k_sums, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->N, c->nblk, c->part);
+    hipLaunchKernelGGL(k_center, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->N, c->nblk, c->part, c->part2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = enqueue_conc_any(c, c->part2)) != hipSuccess) return e;
+    return enqueue_update(c);
+}
+
+// one iteration: replay of the captured graph (captured on first use), or
+// the eager launch sequence when the params ask for no graph
+static hipError_t ensure_graph(scft_ctx *c)
+{
+    if (c->p.use_graph && !c->graph_exec) {
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t e1 = enqueue_iteration(c);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(c->stream, &g);
+        if (e1 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return e1;
+        }
+        if (e != hipSuccess) return e;
+        c->graph = g;
+        e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            c->graph_exec = nullptr;
+            return e;
+        }
+    }
+    return hipSuccess;
+}
+
+static hipError_t launch_iteration(scft_ctx *c)
+{
+    if (!c->p.use_graph) return enqueue_iteration(c);
+    hipError_t e = ensure_graph(c);
+    if (e != hipSuccess) return e;
+    return hipGraphLaunch(c->graph_exec, c->stream);
+}
+
+static int check_fe(scft_ctx *c, const std::vector<double> &fe)
+{
+    for (int b = 0; b < c->batch; ++b)
+        if (!std::isfinite(fe[b * 5 + 0]) || !std::isfinite(fe[b * 5 + 3]))
+            return set_err(c, SCFT_E_NONFINITE,
+                           "non-finite free energy in instance " + std::to_string(b) + " (Q <= 0 or overflow)");
+    return SCFT_OK;
+}
+
+extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *q_init, double *q_out)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (block != 0 && block != 1) return set_err(ctx, SCFT_E_INVALID, "block must be 0 (A) or 1 (B)");
+    if (sign != 1 && sign != -1) return set_err(ctx, SCFT_E_INVALID, "sign must be +1 or -1");
+    if (!q_out) return set_err(ctx, SCFT_E_INVALID, "q_out is NULL");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    const int ns = block == 0 ? ctx->NsA : ctx->NsB;
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    const double *w = block == 0 ? ctx->wA : ctx->wB;
+    double *stage = (double *)ctx->work;  // BN doubles fit in the complex work buffer
+    char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->tsize;
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
+                           ctx->p.ds / 2, (float *)wdz_b);
+    else
+        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
+                           ctx->p.ds / 2, (double *)wdz_b);
+    if (q_init) HIPX(ctx, hipMemcpyAsync(stage, q_init, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
+                           q_init ? stage : nullptr, (long)BN);
+    else
+        hipLaunchKernelGGL(k_seed_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (double2 *)ctx->slots,
+                           q_init ? stage : nullptr, (long)BN);
+    HIPX(ctx, hipGetLastError());
+    hipError_t e = ctx->fp32 ? enqueue_march<float>(ctx, ns, (const float *)wdz_b, (const float *)wdz_b, false)
+                             : enqueue_march<double>(ctx, ns, (const double *)wdz_b, (const double *)wdz_b, false);
+    HIPX(ctx, e);
+    std::vector<double> host(BN);
+    for (int t = 0; t <= ns; ++t) {
+        const char *sl = (const char *)ctx->slots + (size_t)t * BN * ctx->csize;
+        if (ctx->fp32)
+            hipLaunchKernelGGL(k_take_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (const float2 *)sl,
+                               stage, (long)BN);
+        else
+            hipLaunchKernelGGL(k_take_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream,
+                               (const double2 *)sl, stage, (long)BN);
+        HIPX(ctx, hipGetLastError());
+        HIPX(ctx, hipMemcpyAsync(host.data(), stage, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(ctx, hipStreamSynchronize(ctx->stream));
+        const int s = sign == 1 ? t : ns - t;
+        for (int b = 0; b < ctx->batch; ++b)
+            std::memcpy(q_out + ((size_t)b * (ns + 1) + s) * ctx->N, host.data() + (size_t)b * ctx->N,
+                        8 * ctx->N);
+    }
+    ctx->have_conc = 0;  // the slice store no longer holds the chain of the current fields
+    return SCFT_OK;
+}
+
+extern "C" int scft_concentrations(scft_ctx *ctx, double *Q)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    const dim3 g(ctx->nblk, ctx->batch);
+    // getConc's average_value(wa) (cuda_scft.cu:71)
+    hipLaunchKernelGGL(k_sums, g, dim3(RT), 0, ctx->stream, ctx->wA, (const double *)nullptr, ctx->N, ctx->nblk,
+                       ctx->part);
+    HIPX(ctx, hipGetLastError());
+    HIPX(ctx, enqueue_conc_any(ctx, ctx->part));
+    std::vector<double> q(ctx->batch);
+    HIPX(ctx, hipMemcpyAsync(q.data(), ctx->Q, 8 * ctx->batch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < ctx->batch; ++b)
+        if (!std::isfinite(q[b]) || !(q[b] > 0.0))
+            return set_err(ctx, SCFT_E_NONFINITE, "Q of instance " + std::to_string(b) + " is not finite and > 0");
+    if (Q) std::memcpy(Q, q.data(), 8 * ctx->batch);
+    ctx->have_conc = 1;
+    return SCFT_OK;
+}
+
+extern "C" int scft_update_fields(scft_ctx *ctx)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (!ctx->have_conc) return set_err(ctx, SCFT_E_STATE, "scft_update_fields needs scft_concentrations first");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    HIPX(ctx, enqueue_update(ctx));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_conc = 0;
+    ctx->have_fe = 1;
+    return SCFT_OK;
+}
+
+extern "C" int scft_free_energy(scft_ctx *ctx, double *out)
+{
+    if (!ctx || !out) return SCFT_E_INVALID;
+    if (!ctx->have_fe) return set_err(ctx, SCFT_E_STATE, "no free energy yet: run an update or an iteration");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    std::vector<double> fe(5 * ctx->batch);
+    HIPX(ctx, hipMemcpyAsync(fe.data(), ctx->fe, 8 * fe.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, fe.data(), 8 * fe.size());
+    return check_fe(ctx, fe);
+}
+
+extern "C" int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, double *fe_out)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (done) *done = 0;
+    if (n_iter < 0) return set_err(ctx, SCFT_E_INVALID, "n_iter must be >= 0");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    std::vector<double> fe(5 * ctx->batch, 0.0);
+    int it = 0;
+    for (; it < n_iter;) {
+        HIPX(ctx, launch_iteration(ctx));
+        ++it;
+        ctx->have_fe = 1;
+        if (tol > 0.0) {
+            HIPX(ctx, hipMemcpyAsync(fe.data(), ctx->fe, 8 * fe.size(), hipMemcpyDeviceToHost, ctx->stream));
+            HIPX(ctx, hipStreamSynchronize(ctx->stream));
+            int rc = check_fe(ctx, fe);
+            if (rc) {
+                if (done) *done = it;
+                return rc;
+            }
+            double worst = 0.0;
+            for (int b = 0; b < ctx->batch; ++b) worst = std::max(worst, fe[b * 5 + 4]);
+            if (worst < tol) break;
+        }
+    }
+    if (done) *done = it;
+    ctx->have_conc = 0;
+    if (it == 0) return SCFT_OK;
+    HIPX(ctx, hipMemcpyAsync(fe.data(), ctx->fe, 8 * fe.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    if (fe_out) std::memcpy(fe_out, fe.data(), 8 * fe.size());
+    return check_fe(ctx, fe);
+}
+
+extern "C" int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms)
+{
+    if (!ctx || !ms || n_iter < 1) return SCFT_E_INVALID;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    HIPX(ctx, ensure_graph(ctx));  // capture + instantiate outside the timed region
+    HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int it = 0; it < n_iter; ++it) HIPX(ctx, launch_iteration(ctx));
+    HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPX(ctx, hipEventSynchronize(ctx->ev1));
+    HIPX(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    ctx->have_fe = 1;
+    ctx->have_conc = 0;
+    std::vector<double> fe(5 * ctx->batch);
+    HIPX(ctx, hipMemcpy(fe.data(), ctx->fe, 8 * fe.size(), hipMemcpyDeviceToHost));
+    return check_fe(ctx, fe);
+}
+
+template <typename T>
+static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps)
+{
+    using C = typename cplx<T>::type;
+    PassArgs<T> pa;
+    pa.src = (const C *)c->work;
+    pa.dst = (C *)c->work;
+    pa.wre = (const T *)c->wdz;
+    pa.wim = (const T *)c->wdz;
+    pa.tw = (const C *)c->tw[axis];
+    pa.kf = (const T *)c->kf[axis];
+    pa.nlines = (long)c->batch * (c->N / c->n[axis]);
+    pa.N = c->N;
+    pa.nx = c->n[0];
+    pa.ny = c->n[1];
+    pa.nz = c->n[2];
+    pa.pre = 0;
+    pa.post = 0;
+    for (int r = 0; r < reps; ++r) {
+        hipError_t e = launch_pass<T>(axis, c->n[axis], pa, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
+{
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 2) return SCFT_E_INVALID;
+    if (!ctx->active[axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    // start from the current slice 0 so the data are well scaled
+    HIPX(ctx, hipMemcpyAsync(ctx->work, ctx->slots, ctx->csize * BN, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPX(ctx, ctx->fp32 ? enqueue_pass_only<float>(ctx, axis, reps) : enqueue_pass_only<double>(ctx, axis, reps));
+    HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPX(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    HIPX(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t / reps;
+    return SCFT_OK;
+}

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP library against the CPU FP64 oracle on the same seeded
+inputs (sizes the oracle finishes in seconds).  FP64 form: 1e-12 relative on a
+march and on one getConc, 1e-10 on multi-iteration F and phi (the bar stated
+for the reference method); FP32 form: the error is measured against the FP64
+oracle and bounded as written in each test."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(pkg, n, L, chiN, fA, ds, prec=0, **kw):
+    nx, ny, nz = n if isinstance(n, tuple) else (n, n, n)
+    lx, ly, lz = L if isinstance(L, tuple) else (L, L, L)
+    return pkg.default_params(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, chiN=chiN, fA=fA, ds=ds,
+                              precision=prec, **kw)
+
+
+def _oparams(orc, p):
+    return orc.params(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, p.chiN, p.fA, p.ds, pi=p.pi,
+                      wopt=p.wopt, wcmp=p.wcmp)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 16), (32, 32, 32), (16, 1, 1), (24, 20, 1)])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_propagate_fp64(pkg, orc, shape, sign):
+    print("case", shape, sign, flush=True)
+    p = _params(pkg, shape, (3.3, 4.1, 2.7), 15.0, 0.4, 0.01)
+    N = p.nx * p.ny * p.nz
+    wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "lamellar", 1.0, 0.3, 11)
+    q0 = 0.5 + pkg.fields.splitmix_uniform(12, N)
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        for block, w in ((0, wA), (1, wB)):
+            q = s.propagate(block, sign, q0)[0]
+            ns = s.NsA if block == 0 else s.NsB
+            op = _oparams(orc, p)
+            qo = orc.march(op, orc.ktable(op), w.copy(), q0.copy(), ns, sign)
+            assert _rel(q, qo) < 1e-12, (block, _rel(q, qo))
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (12, 20, 48)])
+def test_concentrations_fp64(pkg, orc, shape):
+    print("case", shape, flush=True)
+    p = _params(pkg, shape, 4.0, 18.0, 0.35, 0.01)
+    wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "gyroid", 1.5, 0.2, 21)
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        Q = s.concentrations()[0]
+        phiA, phiB, wAd, _ = s.get_fields()
+    op = _oparams(orc, p)
+    a, b = wA.copy(), wB.copy()
+    pa, pb, Qo = orc.conc(op, orc.ktable(op), a, b)
+    assert abs(Q - Qo) / Qo < 1e-12
+    assert _rel(phiA[0], pa) < 1e-12
+    assert _rel(phiB[0], pb) < 1e-12
+    assert _rel(wAd[0], a) < 1e-14   # getConc's own mean subtraction of wA
+
+
+def test_iterate_fp64_c1(pkg, orc):
+    """c1 physics (32^3, f=0.5, chiN=15, L=4, ds=0.01), 20 iterations."""
+    p = _params(pkg, 32, 4.0, 15.0, 0.5, 0.01)
+    wA, wB = pkg.fields.workload_fields("c1_lamellar_32")
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        F = []
+        for _ in range(4):
+            done, fe = s.iterate(5)
+            F.append(fe[0].copy())
+        phiA, phiB, wAg, wBg = s.get_fields()
+    op = _oparams(orc, p)
+    a, b = wA.copy(), wB.copy()
+    pa, pb, hist = orc.run(op, a, b, 20)
+    for k in range(4):
+        fo = hist[5 * k + 4]
+        assert abs(F[k][0] - fo[0]) / abs(fo[0]) < 1e-10, (k, F[k], fo)
+        assert abs(F[k][4] - fo[4]) < 1e-10
+    assert _rel(phiA[0], pa) < 1e-10
+    assert _rel(wAg[0], a) < 1e-10
+
+
+def test_anderson_fp64(pkg, orc):
+    p = _params(pkg, 16, 4.0, 15.0, 0.5, 0.01, mixing=1, anderson_m=5, anderson_mix=0.1)
+    wA, wB = pkg.fields.seeded_fields(16, 16, 16, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.1, 31)
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        _, fe = s.iterate(30)
+        phiA, _, _, _ = s.get_fields()
+    op = _oparams(orc, p)
+    a, b = wA.copy(), wB.copy()
+    and_ = orc.Anderson(16 ** 3, 5, 0.1, 0.10)
+    kt = orc.ktable(op)
+    pa = np.empty(16 ** 3)
+    pb = np.empty(16 ** 3)
+    for _ in range(30):
+        Q, feo = and_.iterate(op, kt, a, b, pa, pb)
+    assert abs(fe[0][0] - feo[0]) / abs(feo[0]) < 1e-9, (fe[0], feo)
+    assert _rel(phiA[0], pa) < 1e-8
+
+
+def test_fp32_form_error_c1(pkg, orc):
+    """FP32 propagator with FP64 Q/phi/F: error after 20 iterations vs the FP64 oracle."""
+    p = _params(pkg, 32, 4.0, 15.0, 0.5, 0.01, prec=1)
+    wA, wB = pkg.fields.workload_fields("c1_lamellar_32")
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        _, fe = s.iterate(20)
+        phiA, _, _, _ = s.get_fields()
+    op = _oparams(orc, p)
+    pa, pb, hist = orc.run(op, wA.copy(), wB.copy(), 20)
+    dF = abs(fe[0][0] - hist[-1][0])
+    dphi = float(np.max(np.abs(phiA[0] - pa)))
+    print(f"FP32 form c1 20 it: dF={dF:.3e} max|dphi|={dphi:.3e}")
+    assert dF < 1e-5
+    assert dphi < 1e-5
+
+
+@pytest.mark.parametrize("n,fA,ds", [(64, 0.3, 0.005), (128, 0.35, 0.005)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_disordered_kat_full_size(pkg, n, fA, ds, prec):
+    """w = 0: phi uniform, Q = 1, F = chiN f(1-f) + 0.1 chiN (1/2 - f)^2 after the
+    update (F uses the updated w), mean(phiA) = NsA*ds -- at the benchmark sizes."""
+    print("case", n, prec, flush=True)
+    chiN = 20.0
+    p = _params(pkg, n, 4.0, chiN, fA, ds, prec=prec)
+    N = n ** 3
+    with pkg.Solver(p) as s:
+        s.set_fields(np.zeros(N), np.zeros(N))
+        _, fe = s.iterate(2)
+        phiA, phiB, _, _ = s.get_fields()
+        NsA, NsB = s.NsA, s.NsB
+    fa, fb = NsA * ds, NsB * ds
+    tol = 1e-12 if prec == 0 else 2e-6
+    assert np.max(np.abs(phiA - fa)) < tol
+    assert np.max(np.abs(phiB - fb)) < tol
+    # with the reference's counts fa + fb = 1 here, so F = chiN fa fb + 0.1 chiN (fb - fa)^2 / 4 ... (closed form)
+    wa = 0.05 * (chiN * fb + (-chiN / 2) - 0.1 * (1 - fa - fb))
+    wb = 0.05 * (chiN * fa + (-chiN / 2) - 0.1 * (1 - fa - fb))
+    Fexp = chiN * fa * fb - (wa * fa + wb * fb)
+    assert abs(fe[0][0] - Fexp) < (1e-12 if prec == 0 else 1e-5), (fe[0], Fexp)
+
+
+@pytest.mark.parametrize("mixing", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replay_matches_eager(pkg, mixing, prec):
+    """The captured-graph iteration equals the eager launch sequence bit for bit,
+    replay after replay, with eager copies and host syncs between replays."""
+    print("case", mixing, prec, flush=True)
+    kw = dict(mixing=mixing, anderson_m=4, anderson_mix=0.2)
+    pg = _params(pkg, (16, 12, 20), (4.0, 3.5, 4.5), 16.0, 0.4, 0.01, prec=prec, use_graph=1, **kw)
+    pe = _params(pkg, (16, 12, 20), (4.0, 3.5, 4.5), 16.0, 0.4, 0.01, prec=prec, use_graph=0, **kw)
+    wA, wB = pkg.fields.seeded_fields(16, 12, 20, 4.0, 3.5, 4.5, "lamellar", 1.0, 0.1, 41)
+    with pkg.Solver(pg) as sg, pkg.Solver(pe) as se:
+        sg.set_fields(wA, wB)
+        se.set_fields(wA, wB)
+        for k in range(6):
+            _, fg = sg.iterate(1)
+            _, fe = se.iterate(1)
+            g = sg.get_fields()
+            e = se.get_fields()
+            assert np.array_equal(fg, fe), (k, fg, fe)
+            for a, b in zip(g, e):
+                assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64)])
+def test_fp32_single_conc_error(pkg, orc, shape):
+    """One getConc in the FP32 form against the FP64 oracle on the same fields."""
+    p = _params(pkg, shape, 4.2, 20.0, 0.3, 0.005, prec=1)
+    wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "hexagonal", 1.0, 0.05, 2)
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        Q = s.concentrations()[0]
+        phiA, phiB, _, _ = s.get_fields()
+    op = _oparams(orc, p)
+    pa, pb, Qo = orc.conc(op, orc.ktable(op), wA.copy(), wB.copy())
+    eQ = abs(Q - Qo) / Qo
+    ephi = float(np.max(np.abs(phiA[0] - pa)))
+    print(f"FP32 single getConc {shape}: relQ={eQ:.3e} max|dphiA|={ephi:.3e}")
+    assert eQ < 1e-5 and ephi < 1e-5
