@@ -784,10 +784,16 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
         return set_err(nullptr, SCFT_E_INVALID, "both blocks need at least one contour step (fA/ds, (1-fA)/ds >= 1)");
     const int n[3] = {p.nx, p.ny, p.nz};
     for (int a = 0; a < 3; ++a)
+        if (n[a] > 1 && (n[a] & 1))
+            return set_err(nullptr, SCFT_E_UNSUPPORTED,
+                           "odd grid length " + std::to_string(n[a]) + ": the reference's wavenumber table "
+                           "(init_cuda.cu:408-418) is not even in k for odd lengths, so its D2Z/Z2D step is "
+                           "not a real operator there; use even lengths (or 1)");
+    for (int a = 0; a < 3; ++a)
         if (n[a] > 1 && !pass_supported(n[a]))
             return set_err(nullptr, SCFT_E_UNSUPPORTED,
-                           "grid length " + std::to_string(n[a]) + " has no kernel tiling (supported: 1..12 "
-                           "(2,3,5-smooth), 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 128, 144, 160, 192, 256)");
+                           "grid length " + std::to_string(n[a]) + " has no kernel tiling (supported: 1, 2, 4, 6, 8, "
+                           "10, 12, 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 128, 144, 160, 192, 256)");
     if (p.nx == 1 && p.ny == 1 && p.nz == 1) return set_err(nullptr, SCFT_E_INVALID, "grid must have a length > 1");
 
     std::unique_ptr<scft_ctx> c(new scft_ctx());

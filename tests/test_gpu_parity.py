@@ -182,3 +182,54 @@ def test_fp32_single_conc_error(pkg, orc, shape):
     ephi = float(np.max(np.abs(phiA[0] - pa)))
     print(f"FP32 single getConc {shape}: relQ={eQ:.3e} max|dphiA|={ephi:.3e}")
     assert eQ < 1e-5 and ephi < 1e-5
+
+
+def test_c1_300_iterations_vs_golden(pkg):
+    """c1: 32^3 lamellar, chiN=15, f=0.5, L=4, ds=0.01, simple mixing, 300
+    iterations: the five F terms every 10 iterations (the reference's print
+    cadence) and phiA after 300 against the oracle fixture, 1e-10 relative."""
+    import os
+    from conftest import GOLDEN_DIR
+    g = np.load(os.path.join(GOLDEN_DIR, "c1_oracle.npz"))
+    p = _params(pkg, 32, 4.0, 15.0, 0.5, 0.01)
+    wA, wB = pkg.fields.workload_fields("c1_lamellar_32")
+    fe_hist = []
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        for _ in range(30):
+            _, fe = s.iterate(10)
+            fe_hist.append(fe[0].copy())
+        phiA, _, wAg, _ = s.get_fields()
+    fe_hist = np.array(fe_hist)
+    ref = g["fe_every10"]
+    relF = np.max(np.abs(fe_hist[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))
+    print(f"c1 300 it: max rel dF = {relF:.3e}, F300 = {fe_hist[-1][0]:.15g}")
+    assert relF < 1e-10
+    assert np.max(np.abs(fe_hist[:, 1:4] - ref[:, 1:4])) < 1e-10
+    assert _rel(phiA[0], g["phiA"]) < 1e-10
+    assert _rel(wAg[0], g["wA"]) < 1e-10
+
+
+def test_nonfinite_field_rejected(pkg):
+    p = _params(pkg, 8, 2.0, 10.0, 0.5, 0.05)
+    with pkg.Solver(p) as s:
+        w = np.zeros(512)
+        w[3] = np.nan
+        with pytest.raises(pkg.ScftError) as e:
+            s.set_fields(w, np.zeros(512))
+        assert e.value.code == pkg.SCFT_E_NONFINITE
+        with pytest.raises(pkg.ScftError) as e:
+            s.update_fields()  # before any concentrations
+        assert e.value.code == pkg.SCFT_E_STATE
+
+
+def test_q_underflow_reported(pkg):
+    """A field so strong that exp(-w ds/2) underflows makes Q = 0: reported, not returned."""
+    p = _params(pkg, 8, 2.0, 10.0, 0.5, 0.05)
+    with pkg.Solver(p) as s:
+        wA = np.full(512, 1e5)
+        wA[0] = 3e5  # non-uniform so the mean subtraction leaves a large field
+        s.set_fields(wA, np.full(512, 1e5))
+        with pytest.raises(pkg.ScftError) as e:
+            s.concentrations()
+        assert e.value.code == pkg.SCFT_E_NONFINITE
