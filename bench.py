@@ -55,6 +55,9 @@ def run_form(pkg, name, prec, steps, warmup):
     p = workload_params(pkg, name, prec)
     wA, wB = pkg.fields.workload_fields(name)
     s = pkg.Solver(p)
+    choice, tms = s.autotune(20)
+    log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} autotune choice={choice} pass_ms={tms}")
+    s.tuning = (choice, tms)
     s.set_fields(wA, wB)
     log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} warmup {warmup}")
     s.iterate(warmup)
@@ -72,7 +75,13 @@ def pass_roofline(s, prec, reps=50):
     out = {}
     for axis, nm in enumerate("xyz"):
         out[nm] = s.time_pass(axis, reps)
-    dom = max(out, key=out.get)
+    try:
+        out["xy_plane"] = s.time_pass(3, reps)
+    except Exception:
+        pass
+    plane = bool(getattr(s, "tuning", ([0, 0, 0, 0], None))[0][3])
+    used = {k: v for k, v in out.items() if k in (("xy_plane", "z") if plane else ("x", "y", "z"))}
+    dom = max(used, key=used.get)
     bytes_per_launch = 4 * b * s.N  # read + write of the complex field (2b bytes per point each way)
     achieved = bytes_per_launch / (out[dom] * 1e-3) / 1e9
     return dom, out, bytes_per_launch, achieved

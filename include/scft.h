@@ -69,11 +69,16 @@ typedef struct {
     double anderson_mix;  /* Anderson mixing fraction applied to the combined residual */
     double pi;            /* pi used for wavenumbers; 0 selects SCFT_REFERENCE_PI */
     int use_graph;        /* 1: iterations replay one captured HIP graph; 0: eager launches */
+    int xy_plane;         /* -1 auto (scft_autotune decides), 0 separate x and y passes,
+                             1 fused xy-plane kernel (needs nx == ny and a plane that fits LDS) */
+    int tiling;           /* -1 default (one line per thread group until scft_autotune),
+                             0 or 1 force one or two lines per thread group on every pass */
 } scft_params;
 
 /* Fill p with the reference's defaults (ab.txt sample sizes aside):
    FP64, simple mixing, wopt 0.05, wcmp 0.10, pi = SCFT_REFERENCE_PI, batch 1,
-   Anderson m 10 / mix 0.1 (used only with SCFT_MIX_ANDERSON), HIP graph on. */
+   Anderson m 10 / mix 0.1 (used only with SCFT_MIX_ANDERSON), HIP graph on,
+   xy_plane auto. */
 void scft_default_params(scft_params *p);
 
 /* NsA and NsB exactly as the reference truncates them (init_cuda.cu:330-331). Host only. */
@@ -123,10 +128,18 @@ int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, double *fe);
 int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
 
 /* Kernel-level timing for the roofline: runs `reps` back-to-back launches of
-   the propagator axis pass `axis` (0 x, 1 y, 2 z) on the current slice store
+   the propagator axis pass `axis` (0 x, 1 y, 2 z, 3 the fused xy-plane kernel)
+   on a copy of slice 0
    between two HIP events and returns the mean duration in ms.  Leaves the
    solver state unchanged except the scratch field. */
 int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms);
+
+/* Autotuner (no reference counterpart; the reference's launch shapes are fixed
+   by factor_decompose_1024, init_cuda.cu:21-80): times every tiling variant of
+   each axis pass (`reps` launches each) and the fused xy-plane kernel, keeps
+   the fastest.  choice (may be NULL): int[4] = variant of x, y, z pass, plane
+   used (0/1); ms (may be NULL): float[4] = best x, y, z pass and plane times. */
+int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 
 /* Problem sizes: N points, NsA, NsB, device bytes allocated. */
 int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes);

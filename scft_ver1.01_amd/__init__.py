@@ -53,7 +53,7 @@ ABI_SYMBOLS = (
     "scft_default_params", "scft_contour_counts", "scft_create", "scft_destroy",
     "scft_last_error", "scft_set_fields", "scft_get_fields", "scft_propagate",
     "scft_concentrations", "scft_update_fields", "scft_free_energy", "scft_iterate",
-    "scft_time_iterations", "scft_time_pass", "scft_sizes",
+    "scft_time_iterations", "scft_time_pass", "scft_autotune", "scft_sizes",
 )
 
 
@@ -72,7 +72,8 @@ class Params(ctypes.Structure):
         ("batch", ctypes.c_int), ("precision", ctypes.c_int), ("mixing", ctypes.c_int),
         ("wopt", ctypes.c_double), ("wcmp", ctypes.c_double),
         ("anderson_m", ctypes.c_int), ("anderson_mix", ctypes.c_double),
-        ("pi", ctypes.c_double), ("use_graph", ctypes.c_int),
+        ("pi", ctypes.c_double), ("use_graph", ctypes.c_int), ("xy_plane", ctypes.c_int),
+        ("tiling", ctypes.c_int),
     ]
 
     def replace(self, **kw):
@@ -117,6 +118,8 @@ def lib():
                                             ctypes.POINTER(ctypes.c_int), _dp]),
             "scft_time_iterations": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
             "scft_time_pass": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+            "scft_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_float)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
         }
@@ -246,6 +249,14 @@ class Solver:
         ms = ctypes.c_float()
         self._check(lib().scft_time_iterations(self._h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, reps=20):
+        """Pick the fastest pass tiling per axis (and plane fusion); returns
+        (choice[4], ms[4]) = variants of x, y, z, plane flag; their times."""
+        ch = (ctypes.c_int * 4)()
+        ms = (ctypes.c_float * 4)()
+        self._check(lib().scft_autotune(self._h, int(reps), ch, ms))
+        return list(ch), list(ms)
 
     def time_pass(self, axis, reps):
         ms = ctypes.c_float()

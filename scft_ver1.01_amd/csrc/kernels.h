@@ -4,7 +4,7 @@
 // Data layout in HBM (one context, `batch` instances of the same grid):
 //   point index   r = (k*ny + j)*nx + i          (x fastest, as init_cuda.cu:425)
 //   fields        wA, wB, phiA, phiB : double [batch][N]
-//   wdz           T [2][batch][N]       exp(-w*ds/2) of block A (0) and B (1)
+//   wdz           T [4][batch][N]       exp(-w*ds/2) of block A, B; the same / N
 //   slice store   C [Ns+1][batch][N]     z(t) = q(t) + i*qdag(Ns - t)
 //   work          C [batch][N]           the field between the three axis passes
 // where T is the propagator precision (double or float) and C its complex type.
@@ -19,9 +19,10 @@ namespace scft {
 enum Axis { AX_X = 0, AX_Y = 1, AX_Z = 2 };
 
 // One batched 1-D convolution pass along an axis:
-//   line <- IDFT[ kf(k) * DFT[ line ] ]      (unnormalised DFTs; kf carries 1/n)
+//   line <- IDFT[ kf(k) * DFT[ line ] ]      (unnormalised DFTs, kf(0) = 1)
 // pre:  the loaded value is multiplied by (wre, wim) part-wise first,
-// post: the result is multiplied by (wre, wim) part-wise before the store.
+// post: the result is multiplied by (wre_post, wim_post) part-wise before the
+//       store (these carry the 1/N of the inverse transforms).
 template <typename T>
 struct PassArgs {
     using C = typename cplx<T>::type;
@@ -29,6 +30,8 @@ struct PassArgs {
     C *dst;
     const T *wre;
     const T *wim;
+    const T *wre_post;
+    const T *wim_post;
     const C *tw;   // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
     const T *kf;   // n entries
     long nlines;   // lines in this launch (batch * N / n)
@@ -39,9 +42,19 @@ struct PassArgs {
 
 // true if the pass kernels have an instantiation for line length n
 bool pass_supported(int n);
-// launch one pass along `axis` (n = length of that axis); returns hipError_t
+// launch one pass along `axis` (n = length of that axis) in tiling variant
+// `variant` (0..PASS_VARIANTS-1: one or two lines per thread group)
+constexpr int PASS_VARIANTS = 2;
 template <typename T>
-hipError_t launch_pass(int axis, int n, const PassArgs<T> &a, hipStream_t s);
+hipError_t launch_pass(int axis, int n, int variant, const PassArgs<T> &a, hipStream_t s);
+// true if an xy-plane kernel exists for nx = ny = n in precision T (LDS fit)
+template <typename T>
+bool plane_supported(int n);
+// x and y passes of every plane (nplanes = batch * nz) in one launch; `a`
+// carries the x tables, twy/kfy the y tables
+template <typename T>
+hipError_t launch_plane(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy, const T *kfy, long nplanes,
+                        hipStream_t s);
 // the (A, B, lines-per-workgroup, threads) tiling used for (axis, n)
 void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *threads);
 

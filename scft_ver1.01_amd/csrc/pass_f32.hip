@@ -2,5 +2,8 @@
 #include "pass_kernels.inc"
 
 namespace scft {
-template hipError_t launch_pass<float>(int, int, const PassArgs<float> &, hipStream_t);
+template hipError_t launch_pass<float>(int, int, int, const PassArgs<float> &, hipStream_t);
+template bool plane_supported<float>(int);
+template hipError_t launch_plane<float>(int, const PassArgs<float> &, const cplx<float>::type *, const float *, long,
+                                        hipStream_t);
 }  // namespace scft

@@ -3,7 +3,10 @@
 
 namespace scft {
 
-template hipError_t launch_pass<double>(int, int, const PassArgs<double> &, hipStream_t);
+template hipError_t launch_pass<double>(int, int, int, const PassArgs<double> &, hipStream_t);
+template bool plane_supported<double>(int);
+template hipError_t launch_plane<double>(int, const PassArgs<double> &, const cplx<double>::type *, const double *, long,
+                                        hipStream_t);
 
 bool pass_supported(int n)
 {

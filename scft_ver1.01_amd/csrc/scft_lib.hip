@@ -145,11 +145,16 @@ __global__ void __launch_bounds__(RT) k_center(double *__restrict__ f0, double *
 }
 
 // getConc's own mean subtraction of wA (if part != null), the W = exp(-w ds/2)
-// factors of both blocks (initilize_wdz, cuda_aid.cu:254), and slice 0 = 1 + i
+// factors of both blocks (initilize_wdz, cuda_aid.cu:254), and slice 0 = 1 + i.
+// wdz layout [4][batch][N]: W_A, W_B (applied before the first pass of a step)
+// and W_A*post, W_B*post (after the last pass), post = 1/N times the factor of
+// the untransformed length-1 axes -- formed in FP64 and rounded once, so the
+// k = 0 mode of every pass is multiplied by exactly 1.
 template <typename T>
 __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const double *__restrict__ wB, long N, int nblk,
-                                             const double *__restrict__ part, double ds2, T *__restrict__ wdz,
-                                             typename cplx<T>::type *__restrict__ slot0, int batch)
+                                             const double *__restrict__ part, double ds2, double post,
+                                             T *__restrict__ wdz, typename cplx<T>::type *__restrict__ slot0,
+                                             int batch)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     using C = typename cplx<T>::type;
@@ -163,18 +168,25 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
             a = a - s0 / (double)N;
             wA[i] = a;
         }
-        wdz[i] = (T)exp(-a * ds2);
-        wdz[BN + i] = (T)exp(-wB[i] * ds2);
+        const double ea = exp(-a * ds2), eb = exp(-wB[i] * ds2);
+        wdz[i] = (T)ea;
+        wdz[BN + i] = (T)eb;
+        wdz[2 * BN + i] = (T)(ea * post);
+        wdz[3 * BN + i] = (T)(eb * post);
         if (slot0) slot0[i] = C{T(1), T(1)};
     }
 }
 
-// W of one block only (sovDifFft computes it from the field as given)
+// W (pre) and W*post of one block only (sovDifFft computes W from the field as given)
 template <typename T>
-__global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, double ds2, T *__restrict__ wdz)
+__global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, double ds2, double post,
+                                                T *__restrict__ wpre, T *__restrict__ wpost)
 {
-    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
-        wdz[i] = (T)exp(-w[i] * ds2);
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) {
+        const double e = exp(-w[i] * ds2);
+        wpre[i] = (T)e;
+        wpost[i] = (T)(e * post);
+    }
 }
 
 // slice <- (q_init or 1) + 0i
@@ -582,12 +594,15 @@ struct scft_ctx {
     int n[3] = {1, 1, 1};
     int active[3] = {0, 0, 0};
     int first_axis = -1, last_axis = -1;
+    int use_plane = 0;  // x and y passes fused in one xy-plane kernel
+    int pass_variant[3] = {0, 0, 0};  // tiling variant per axis (autotuned)
     int nblk = 1;
     size_t csize = 16;  // bytes per complex element
     size_t tsize = 8;   // bytes per real element
     // device buffers
     double *wA = nullptr, *wB = nullptr, *phiA = nullptr, *phiB = nullptr;
-    void *wdz = nullptr;    // T [2][batch][N]
+    void *wdz = nullptr;    // T [4][batch][N]: W_A, W_B, W_A*post, W_B*post
+    double post = 1.0;      // 1/N times the factor of the length-1 axes
     void *slots = nullptr;  // C [Ns+1][batch][N]
     void *work = nullptr;   // C [batch][N]
     double *part = nullptr, *part2 = nullptr, *fpart = nullptr;
@@ -656,6 +671,8 @@ extern "C" void scft_default_params(scft_params *p)
     p->anderson_mix = 0.1;
     p->pi = SCFT_REFERENCE_PI;
     p->use_graph = 1;
+    p->xy_plane = -1;
+    p->tiling = -1;
 }
 
 // init_cuda.cu:326-331
@@ -715,17 +732,18 @@ static int upload_tables(scft_ctx *c)
     std::vector<double> k2[3];
     for (int a = 0; a < 3; ++a) ksq_axis(c->n[a], ls[a], pi, k2[a]);
     // axes of length 1 are not transformed; their single factor exp(-ds k^2[0])
-    // (nonzero: the reference's table gives k = -2 pi/l there) folds into the first pass
+    // (nonzero: the reference's table gives k = -2 pi/l there) and the inverse
+    // transforms' 1/N go into the post-step factor, so kf[0] = 1 exactly
     double fold = 1.0;
     for (int a = 0; a < 3; ++a)
         if (!c->active[a]) fold *= std::exp(-c->p.ds * k2[a][0]);
+    c->post = fold / (double)c->N;
     for (int a = 0; a < 3; ++a) {
         if (!c->active[a]) continue;
         const int n = c->n[a];
         std::vector<double> kf(n), twr(n), twi(n);
         for (int k = 0; k < n; ++k) {
-            kf[k] = std::exp(-c->p.ds * k2[a][k]) / n;
-            if (a == c->first_axis) kf[k] *= fold;
+            kf[k] = std::exp(-c->p.ds * k2[a][k]);
             // exact-angle twiddles, long double then rounded
             const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
             twr[k] = (double)cosl(ang);
@@ -816,6 +834,17 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
         }
     }
     // reduction blocks per instance: ~2 points per thread, at most 2048 partials
+    {
+        const bool fits = p.nx == p.ny && p.nx > 1 &&
+                          (c->fp32 ? plane_supported<float>(p.nx) : plane_supported<double>(p.nx));
+        if (p.xy_plane == 1 && !fits)
+            return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane = 1 needs nx == ny with a plane kernel that "
+                                                        "fits LDS (16, 32, 48, 64; 128 in FP32)");
+        c->use_plane = (p.xy_plane == 1) && fits;  // auto: decided by scft_autotune
+    }
+    if (p.tiling < -1 || p.tiling >= PASS_VARIANTS)
+        return set_err(nullptr, SCFT_E_INVALID, "tiling must be -1 or 0.." + std::to_string(PASS_VARIANTS - 1));
+    for (int a = 0; a < 3; ++a) c->pass_variant[a] = p.tiling < 0 ? 0 : p.tiling;
     c->nblk = (int)std::min<long>(2048, std::max<long>(1, (c->N + RT * 2 - 1) / (RT * 2)));
 
     int ndev = 0;
@@ -843,7 +872,7 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     A((void **)&c->wB, 8 * BN);
     A((void **)&c->phiA, 8 * BN);
     A((void **)&c->phiB, 8 * BN);
-    A(&c->wdz, 2 * c->tsize * BN);
+    A(&c->wdz, 4 * c->tsize * BN);
     A(&c->slots, (size_t)(c->Ns + 1) * c->csize * BN);
     A(&c->work, c->csize * BN);
     A((void **)&c->part, sizeof(double) * 2 * c->batch * c->nblk);
@@ -938,31 +967,38 @@ extern "C" int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double
 // ---------------------------------------------------------------------------
 static unsigned pt_grid(long n) { return (unsigned)std::min<long>(4096, std::max<long>(1, (n + RT - 1) / RT)); }
 
+// march `nsteps` packed steps from slice 0; per step either three axis
+// passes or (c->use_plane) the fused xy-plane kernel and the z pass
+// block: -1 packed chain (real part q with A then B, imaginary part qdag with
+// B then A), 0 or 1 a single block's W on both parts (sovDifFft's march)
 template <typename T>
-static hipError_t enqueue_march(scft_ctx *c, int nsteps, const T *wre_t, const T *wim_t, bool packed)
+static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block)
 {
     using C = typename cplx<T>::type;
     const size_t BN = (size_t)c->batch * c->N;
     C *slots = (C *)c->slots;
     C *work = (C *)c->work;
-    const T *wdzA = (const T *)c->wdz;
-    const T *wdzB = wdzA + BN;
+    const T *W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
     for (int t = 1; t <= nsteps; ++t) {
-        const T *wre, *wim;
-        if (packed) {
-            wre = (t <= c->NsA) ? wdzA : wdzB;  // q(t): block A for t <= NsA
-            wim = (t <= c->NsB) ? wdzB : wdzA;  // qdag(Ns-t): block B for t <= NsB
+        int br, bi;  // block of the real and imaginary part at step t
+        if (block < 0) {
+            br = (t <= c->NsA) ? 0 : 1;  // q(t): block A for t <= NsA
+            bi = (t <= c->NsB) ? 1 : 0;  // qdag(Ns-t): block B for t <= NsB
         } else {
-            wre = wre_t;
-            wim = wim_t;
+            br = bi = block;
         }
-        for (int a = 0; a < 3; ++a) {
-            if (!c->active[a]) continue;
+        const T *wre = W + br * BN, *wim = W + bi * BN;
+        const T *pre_r = W + (2 + br) * BN, *pre_i = W + (2 + bi) * BN;
+        const C *src = slots + (size_t)(t - 1) * BN;
+        C *dst = slots + (size_t)t * BN;
+        auto args = [&](int a, bool first, bool last) {
             PassArgs<T> pa;
-            pa.src = (a == c->first_axis) ? slots + (size_t)(t - 1) * BN : work;
-            pa.dst = (a == c->last_axis) ? slots + (size_t)t * BN : work;
+            pa.src = first ? src : work;
+            pa.dst = last ? dst : work;
             pa.wre = wre;
             pa.wim = wim;
+            pa.wre_post = pre_r;
+            pa.wim_post = pre_i;
             pa.tw = (const C *)c->tw[a];
             pa.kf = (const T *)c->kf[a];
             pa.nlines = (long)c->batch * (c->N / c->n[a]);
@@ -970,9 +1006,26 @@ static hipError_t enqueue_march(scft_ctx *c, int nsteps, const T *wre_t, const T
             pa.nx = c->n[0];
             pa.ny = c->n[1];
             pa.nz = c->n[2];
-            pa.pre = (a == c->first_axis);
-            pa.post = (a == c->last_axis);
-            hipError_t e = launch_pass<T>(a, c->n[a], pa, c->stream);
+            pa.pre = first;
+            pa.post = last;
+            return pa;
+        };
+        if (c->use_plane) {
+            const bool zact = c->active[2];
+            PassArgs<T> pa = args(0, true, !zact);
+            hipError_t e = launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], (const T *)c->kf[1],
+                                           (long)c->batch * c->n[2], c->stream);
+            if (e != hipSuccess) return e;
+            if (zact) {
+                e = launch_pass<T>(2, c->n[2], c->pass_variant[2], args(2, false, true), c->stream);
+                if (e != hipSuccess) return e;
+            }
+            continue;
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (!c->active[a]) continue;
+            hipError_t e = launch_pass<T>(a, c->n[a], c->pass_variant[a], args(a, a == c->first_axis,
+                                                                             a == c->last_axis), c->stream);
             if (e != hipSuccess) return e;
         }
     }
@@ -987,10 +1040,10 @@ static hipError_t enqueue_conc(scft_ctx *c, const double *mean_part)
     const size_t BN = (size_t)c->batch * c->N;
     const dim3 g(c->nblk, c->batch);
     hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->N, c->nblk, mean_part,
-                       c->p.ds / 2, (T *)c->wdz, (C *)c->slots, c->batch);
+                       c->p.ds / 2, c->post, (T *)c->wdz, (C *)c->slots, c->batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = enqueue_march<T>(c, c->Ns, nullptr, nullptr, true)) != hipSuccess) return e;
+    if ((e = enqueue_march<T>(c, c->Ns, -1)) != hipSuccess) return e;
     const C *slotNs = (const C *)c->slots + (size_t)c->Ns * BN;
     hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, slotNs, c->N, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
@@ -1099,12 +1152,13 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
     const double *w = block == 0 ? ctx->wA : ctx->wB;
     double *stage = (double *)ctx->work;  // BN doubles fit in the complex work buffer
     char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->tsize;
+    char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->tsize;
     if (ctx->fp32)
         hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
-                           ctx->p.ds / 2, (float *)wdz_b);
+                           ctx->p.ds / 2, ctx->post, (float *)wdz_b, (float *)wpost_b);
     else
         hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
-                           ctx->p.ds / 2, (double *)wdz_b);
+                           ctx->p.ds / 2, ctx->post, (double *)wdz_b, (double *)wpost_b);
     if (q_init) HIPX(ctx, hipMemcpyAsync(stage, q_init, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
     if (ctx->fp32)
         hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
@@ -1113,8 +1167,7 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
         hipLaunchKernelGGL(k_seed_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (double2 *)ctx->slots,
                            q_init ? stage : nullptr, (long)BN);
     HIPX(ctx, hipGetLastError());
-    hipError_t e = ctx->fp32 ? enqueue_march<float>(ctx, ns, (const float *)wdz_b, (const float *)wdz_b, false)
-                             : enqueue_march<double>(ctx, ns, (const double *)wdz_b, (const double *)wdz_b, false);
+    hipError_t e = ctx->fp32 ? enqueue_march<float>(ctx, ns, block) : enqueue_march<double>(ctx, ns, block);
     HIPX(ctx, e);
     std::vector<double> host(BN);
     for (int t = 0; t <= ns; ++t) {
@@ -1234,7 +1287,7 @@ extern "C" int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms)
 }
 
 template <typename T>
-static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps)
+static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant)
 {
     using C = typename cplx<T>::type;
     PassArgs<T> pa;
@@ -1242,9 +1295,12 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps)
     pa.dst = (C *)c->work;
     pa.wre = (const T *)c->wdz;
     pa.wim = (const T *)c->wdz;
-    pa.tw = (const C *)c->tw[axis];
-    pa.kf = (const T *)c->kf[axis];
-    pa.nlines = (long)c->batch * (c->N / c->n[axis]);
+    pa.wre_post = (const T *)c->wdz;
+    pa.wim_post = (const T *)c->wdz;
+    const int ax = axis == 3 ? 0 : axis;
+    pa.tw = (const C *)c->tw[ax];
+    pa.kf = (const T *)c->kf[ax];
+    pa.nlines = (long)c->batch * (c->N / c->n[ax]);
     pa.N = c->N;
     pa.nx = c->n[0];
     pa.ny = c->n[1];
@@ -1252,26 +1308,108 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps)
     pa.pre = 0;
     pa.post = 0;
     for (int r = 0; r < reps; ++r) {
-        hipError_t e = launch_pass<T>(axis, c->n[axis], pa, c->stream);
+        hipError_t e = axis == 3 ? launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], (const T *)c->kf[1],
+                                                   (long)c->batch * c->n[2], c->stream)
+                                 : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
+// mean duration (ms) of `reps` back-to-back launches of one pass kernel on a
+// field of ones in the work buffer
+static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, float *ms)
 {
-    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 2) return SCFT_E_INVALID;
-    if (!ctx->active[axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
-    HIPX(ctx, hipSetDevice(ctx->device));
     const size_t BN = (size_t)ctx->batch * ctx->N;
-    // start from the current slice 0 so the data are well scaled
-    HIPX(ctx, hipMemcpyAsync(ctx->work, ctx->slots, ctx->csize * BN, hipMemcpyDeviceToDevice, ctx->stream));
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->work,
+                           (const double *)nullptr, (long)BN);
+    else
+        hipLaunchKernelGGL(k_seed_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (double2 *)ctx->work,
+                           (const double *)nullptr, (long)BN);
+    HIPX(ctx, hipGetLastError());
+    // one untimed launch so code loading and first-touch costs stay outside
+    HIPX(ctx, ctx->fp32 ? enqueue_pass_only<float>(ctx, axis, 1, variant)
+                        : enqueue_pass_only<double>(ctx, axis, 1, variant));
     HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIPX(ctx, ctx->fp32 ? enqueue_pass_only<float>(ctx, axis, reps) : enqueue_pass_only<double>(ctx, axis, reps));
+    HIPX(ctx, ctx->fp32 ? enqueue_pass_only<float>(ctx, axis, reps, variant)
+                        : enqueue_pass_only<double>(ctx, axis, reps, variant));
     HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIPX(ctx, hipEventSynchronize(ctx->ev1));
     float t = 0.0f;
     HIPX(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
     *ms = t / reps;
+    return SCFT_OK;
+}
+
+extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
+{
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 3) return SCFT_E_INVALID;
+    if (axis == 3 && !(ctx->n[0] == ctx->n[1] &&
+                       (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]))))
+        return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel for this grid");
+    if (axis < 3 && !ctx->active[axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    return time_pass_variant(ctx, axis, reps, axis < 3 ? ctx->pass_variant[axis] : 0, ms);
+}
+
+// Autotuner: time every tiling variant of every active axis pass (and the
+// fused xy-plane kernel where it fits) on this context's grid, keep the
+// fastest combination, and drop the captured graph so the next iteration
+// re-captures with the new choice.
+extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out)
+{
+    if (!ctx || reps < 1) return SCFT_E_INVALID;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    float best_sep = 0.0f;
+    float per_axis[3] = {0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < 3; ++a) {
+        if (!ctx->active[a]) continue;
+        int bv = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < PASS_VARIANTS; ++v) {
+            float t = 0.0f;
+            int rc = time_pass_variant(ctx, a, reps, v, &t);
+            if (rc == SCFT_E_HIP && v > 0) {  // tiling does not fit this size: skip it
+                (void)hipGetLastError();
+                continue;
+            }
+            if (rc) return rc;
+            if (t < bt) {
+                bt = t;
+                bv = v;
+            }
+        }
+        ctx->pass_variant[a] = bv;
+        per_axis[a] = bt;
+        best_sep += bt;
+    }
+    int plane = 0;
+    const bool fits = ctx->n[0] == ctx->n[1] && ctx->n[0] > 1 &&
+                      (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]));
+    float tplane = 0.0f;
+    if (fits && ctx->p.xy_plane != 0) {
+        int rc = time_pass_variant(ctx, 3, reps, 0, &tplane);
+        if (rc) return rc;
+        const float with_plane = tplane + (ctx->active[2] ? per_axis[2] : 0.0f);
+        plane = (ctx->p.xy_plane == 1) || with_plane < best_sep;
+    }
+    ctx->use_plane = plane;
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    if (choice) {
+        choice[0] = ctx->pass_variant[0];
+        choice[1] = ctx->pass_variant[1];
+        choice[2] = ctx->pass_variant[2];
+        choice[3] = plane;
+    }
+    if (ms_out) {
+        ms_out[0] = per_axis[0];
+        ms_out[1] = per_axis[1];
+        ms_out[2] = per_axis[2];
+        ms_out[3] = tplane;
+    }
     return SCFT_OK;
 }

@@ -233,3 +233,62 @@ def test_q_underflow_reported(pkg):
         with pytest.raises(pkg.ScftError) as e:
             s.concentrations()
         assert e.value.code == pkg.SCFT_E_NONFINITE
+
+
+@pytest.mark.parametrize("shape,prec", [((32, 32, 32), 0), ((64, 64, 8), 0), ((48, 48, 6), 1),
+                                        ((128, 128, 4), 1), ((64, 64, 1), 0), ((16, 16, 24), 1)])
+def test_xy_plane_kernel(pkg, orc, shape, prec):
+    """The fused xy-plane march equals the separate-pass march and the oracle."""
+    print("case", shape, prec, flush=True)
+    out = {}
+    for plane in (0, 1):
+        p = _params(pkg, shape, (4.0, 4.0, 3.0), 16.0, 0.35, 0.02, prec=prec, xy_plane=plane)
+        wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "gyroid", 1.2, 0.2, 51)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations()[0]
+            phiA = s.get_fields()[0][0]
+        out[plane] = (Q, phiA)
+    op = _oparams(orc, p)
+    pa, pb, Qo = orc.conc(op, orc.ktable(op), wA.copy(), wB.copy())
+    tol = 1e-12 if prec == 0 else 2e-5
+    for plane in (0, 1):
+        eQ, ephi = abs(out[plane][0] - Qo) / Qo, _rel(out[plane][1], pa)
+        print(f"  plane={plane}: relQ={eQ:.3e} rel_phiA={ephi:.3e}")
+        assert eQ < tol and ephi < tol, plane
+
+
+@pytest.mark.parametrize("tiling", [1])
+@pytest.mark.parametrize("shape,prec", [((32, 32, 32), 0), ((24, 20, 36), 1), ((64, 64, 8), 1), ((2, 10, 6), 1),
+                                        ((128, 16, 12), 1), ((20, 36, 8), 0)])
+def test_tiling_variants_identical(pkg, shape, prec, tiling):
+    """Every pass tiling variant does the same arithmetic per line (up to the
+    compiler's FMA contraction choices): equal to the default to rounding,
+    including ragged line counts (the last workgroup partly idle)."""
+    print("case", shape, prec, tiling, flush=True)
+    res = []
+    for t in (0, tiling):
+        p = _params(pkg, shape, 4.0, 15.0, 0.4, 0.02, prec=prec, tiling=t, xy_plane=0)
+        wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.2, 61)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            res.append((s.concentrations()[0], s.get_fields()[0]))
+    tol = 1e-13 if prec == 0 else 1e-6
+    assert abs(res[0][0] - res[1][0]) / res[0][0] < tol
+    assert _rel(res[1][1], res[0][1]) < tol
+
+
+def test_autotune_keeps_results(pkg):
+    p = _params(pkg, 32, 4.0, 15.0, 0.4, 0.02, prec=1)
+    wA, wB = pkg.fields.seeded_fields(32, 32, 32, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.2, 62)
+    out = []
+    for tune in (False, True):
+        with pkg.Solver(p) as s:
+            if tune:
+                choice, ms = s.autotune(5)
+                print("autotune", choice, ms)
+            s.set_fields(wA, wB)
+            _, fe = s.iterate(5)
+            out.append(fe)
+    # plane fusion and tiling change the order of operations only within FP32 rounding
+    assert np.max(np.abs(out[0] - out[1])) < 2e-6
