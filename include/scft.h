@@ -73,6 +73,9 @@ typedef struct {
                              1 fused xy-plane kernel (needs nx == ny and a plane that fits LDS) */
     int tiling;           /* -1 default (one line per thread group until scft_autotune),
                              0 or 1 force one or two lines per thread group on every pass */
+    int checkpoint;       /* 0: keep every q(s) slice; 1: sqrt(Ns) checkpointing -- keep
+                             z(kK), K = ceil(sqrt(Ns+1)), recompute segments (about 2x the
+                             propagator steps, O(sqrt Ns) slices of memory) */
 } scft_params;
 
 /* Fill p with the reference's defaults (ab.txt sample sizes aside):
@@ -133,6 +136,30 @@ int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
    between two HIP events and returns the mean duration in ms.  Leaves the
    solver state unchanged except the scratch field. */
 int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms);
+
+/* Discrete adjoint of one chain evaluation (no reference counterpart; the
+   closest reference idea is the unused targeted update phi_w_constrained,
+   cuda_aid.cu:418-452).  For fields uA, uB (host [batch][N], used as given:
+   no mean subtraction) computes phi = getConc(u), Q, and
+   L = sum_r (lamA phiA + lamB phiB) + mu ln Q per instance, and its gradient
+   gA = dL/duA, gB = dL/duB by reverse mode through the march, the trapezoid
+   sum and Q.  lamA/lamB may be NULL (zero).  Outputs may be NULL.  With
+   lam = 0, mu = 1 the gradient is dlnQ/du = -phi/N. */
+int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *uB, const double *lamA,
+                      const double *lamB, double mu, double *L, double *phiA, double *phiB, double *Q,
+                      double *gA, double *gB);
+
+/* Gradient of a structure-targeting loss through the unrolled SCFT loop (no
+   reference counterpart).  External field h (host [batch][N]) acts on the A
+   blocks: iteration k evaluates the chain at (P wA + h, P wB) (P removes the
+   mean), and between evaluations updates w with the context's mixing (simple
+   or Anderson; FP64 on the host).  Starting from (w0A, w0B), n_iter chain
+   evaluations are made; loss[b] = sum_r (phiA - target)^2 of the last one.
+   grad_h (may be NULL: forward only) receives dloss/dh by the discrete
+   adjoint of every step (mixing, least squares, chain, mean removal).
+   phiA_last (may be NULL) receives the last phiA. */
+int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const double *w0B, int n_iter,
+                      const double *target, double *loss, double *grad_h, double *phiA_last);
 
 /* Autotuner (no reference counterpart; the reference's launch shapes are fixed
    by factor_decompose_1024, init_cuda.cu:21-80): times every tiling variant of

@@ -55,6 +55,8 @@ def lib():
             "orc_free_energy": (None, [P, ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]),
             "orc_iterate": (ctypes.c_double, [P, _dp, _dp, _dp, _dp, _dp, _dp]),
             "orc_run": (None, [P, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]),
+            "orc_conc_adjoint": (ctypes.c_double, [P, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp,
+                                                   _dp, _dp, _dp]),
             "orc_anderson_create": (ctypes.c_void_p, [ctypes.c_long, ctypes.c_int,
                                                       ctypes.c_double, ctypes.c_double]),
             "orc_anderson_destroy": (None, [ctypes.c_void_p]),
@@ -170,6 +172,18 @@ def conc(p, kt, wA, wB, keep_q=False):
     Q = lib().orc_conc(ctypes.byref(p), _p(kt), _p(wA), _p(wB), _p(phiA), _p(phiB),
                        None, None, None, None)
     return phiA, phiB, Q
+
+
+def conc_adjoint(p, kt, uA, uB, lamA, lamB, mu=0.0):
+    """L = sum(lamA phiA + lamB phiB) + mu ln Q for fields used as given;
+    returns (L, gA, gB, phiA, phiB, Q) with g = dL/du."""
+    N = p.nx * p.ny * p.nz
+    phiA = np.empty(N); phiB = np.empty(N); gA = np.empty(N); gB = np.empty(N)
+    Q = ctypes.c_double()
+    L = lib().orc_conc_adjoint(ctypes.byref(p), _p(kt), _p(np.ascontiguousarray(uA)), _p(np.ascontiguousarray(uB)),
+                               _p(np.ascontiguousarray(lamA)), _p(np.ascontiguousarray(lamB)), mu,
+                               _p(phiA), _p(phiB), ctypes.byref(Q), _p(gA), _p(gB))
+    return L, gA, gB, phiA, phiB, Q.value
 
 
 def phi_w(p, wA, wB, phiA, phiB):

@@ -521,6 +521,103 @@ void orc_run(const orc_params *p, double *wA, double *wB, double *phiA, double *
 }
 
 /* ------------------------------------------------------------------------ */
+/* Discrete adjoint of one chain evaluation (no reference counterpart).       */
+/* Forward: q(0) = 1, q(j) = S_b(j) q(j-1); qd(Ns) = 1, qd(j-1) = S_b(j) qd(j), */
+/* S_b = W_b K W_b with W_b = exp(-u_b ds/2), b(j) = A for j <= NsA;          */
+/* Q = mean q(Ns); phi_b = (ds/Q) trap_b[q qd].                              */
+/* L = sum(lamA phiA + lamB phiB) + mu ln Q.  With e(s) = dL/dq(s) and        */
+/* ed(s) = dL/dqd(s) (explicit parts), S symmetric gives                     */
+/*   qb(Ns) = e(Ns), qb(j-1) = e(j-1) + S_b(j) qb(j)                          */
+/*   qdb(0) = ed(0), qdb(j) = ed(j) + S_b(j) qdb(j-1)                         */
+/* and, per step j of block b (dW = -(ds/2) W du):                            */
+/*   du_b += -(ds/2)[qb(j) q(j) + q(j-1)(qb(j-1) - e(j-1))                    */
+/*                  + qdb(j-1) qd(j-1) + qd(j)(qdb(j) - ed(j))]               */
+/* ------------------------------------------------------------------------ */
+
+double orc_conc_adjoint(const orc_params *p, const double *kxyzdz, const double *uA, const double *uB,
+                        const double *lamA, const double *lamB, double mu, double *phiA, double *phiB,
+                        double *Qout, double *gA, double *gB)
+{
+    const long N = npts(p);
+    int NsA, NsB, Ns, j;
+    long r;
+    double Q, L, dLdQ, lphi = 0.0;
+    orc_plan3 p3;
+    double *q, *qd, *qb, *qdb, *e, *ed, *WA, *WB, *tmp, *buf;
+    orc_contour_counts(p->fA, p->ds, &NsA, &NsB);
+    Ns = NsA + NsB;
+    q = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    qd = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    qb = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    qdb = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    e = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    ed = (double *)malloc(sizeof(double) * (size_t)N * (Ns + 1));
+    WA = (double *)malloc(sizeof(double) * (size_t)N);
+    WB = (double *)malloc(sizeof(double) * (size_t)N);
+    tmp = (double *)malloc(sizeof(double) * (size_t)N);
+    buf = (double *)malloc(sizeof(double) * (size_t)N * 2);
+    plan3_init(&p3, p->nx, p->ny, p->nz);
+    for (r = 0; r < N; r++) {
+        WA[r] = exp(-uA[r] * p->ds / 2);
+        WB[r] = exp(-uB[r] * p->ds / 2);
+    }
+#define WOF(jj) ((jj) <= NsA ? WA : WB)
+    for (r = 0; r < N; r++) { q[r] = 1.0; qd[(long)Ns * N + r] = 1.0; }
+    for (j = 1; j <= Ns; j++)
+        step_with_plan(p, &p3, kxyzdz, WOF(j), q + (long)(j - 1) * N, q + (long)j * N, buf, buf + N);
+    for (j = Ns; j >= 1; j--)
+        step_with_plan(p, &p3, kxyzdz, WOF(j), qd + (long)j * N, qd + (long)(j - 1) * N, buf, buf + N);
+    Q = 0.0;
+    for (r = 0; r < N; r++) Q += q[(long)Ns * N + r];
+    Q /= N;
+    for (r = 0; r < N; r++) {
+        double a = 0.0, b = 0.0;
+        int s;
+        for (s = 0; s <= NsA; s++) a += ((s == 0 || s == NsA) ? 0.5 : 1.0) * q[(long)s * N + r] * qd[(long)s * N + r];
+        for (s = NsA; s <= Ns; s++) b += ((s == NsA || s == Ns) ? 0.5 : 1.0) * q[(long)s * N + r] * qd[(long)s * N + r];
+        phiA[r] = a * p->ds / Q;
+        phiB[r] = b * p->ds / Q;
+        lphi += lamA[r] * phiA[r] + lamB[r] * phiB[r];
+    }
+    L = lphi + mu * log(Q);
+    dLdQ = -lphi / Q + mu / Q;
+    /* explicit parts: dphi_b(r)/dq(s)(r) = (ds/Q) c_b(s) qd(s)(r) */
+    for (j = 0; j <= Ns; j++) {
+        const double cA = (j <= NsA) ? ((j == 0 || j == NsA) ? 0.5 : 1.0) : 0.0;
+        const double cB = (j >= NsA) ? ((j == NsA || j == Ns) ? 0.5 : 1.0) : 0.0;
+        for (r = 0; r < N; r++) {
+            const double g = p->ds / Q * (cA * lamA[r] + cB * lamB[r]);
+            e[(long)j * N + r] = g * qd[(long)j * N + r] + (j == Ns ? dLdQ / N : 0.0);
+            ed[(long)j * N + r] = g * q[(long)j * N + r];
+        }
+    }
+    memcpy(qb + (long)Ns * N, e + (long)Ns * N, sizeof(double) * (size_t)N);
+    for (j = Ns; j >= 1; j--) {
+        step_with_plan(p, &p3, kxyzdz, WOF(j), qb + (long)j * N, tmp, buf, buf + N);
+        for (r = 0; r < N; r++) qb[(long)(j - 1) * N + r] = e[(long)(j - 1) * N + r] + tmp[r];
+    }
+    memcpy(qdb, ed, sizeof(double) * (size_t)N);
+    for (j = 1; j <= Ns; j++) {
+        step_with_plan(p, &p3, kxyzdz, WOF(j), qdb + (long)(j - 1) * N, tmp, buf, buf + N);
+        for (r = 0; r < N; r++) qdb[(long)j * N + r] = ed[(long)j * N + r] + tmp[r];
+    }
+    for (r = 0; r < N; r++) { gA[r] = 0.0; gB[r] = 0.0; }
+    for (j = 1; j <= Ns; j++) {
+        double *g = (j <= NsA) ? gA : gB;
+        const long o1 = (long)j * N, o0 = (long)(j - 1) * N;
+        for (r = 0; r < N; r++)
+            g[r] += -(p->ds / 2) * (qb[o1 + r] * q[o1 + r] + q[o0 + r] * (qb[o0 + r] - e[o0 + r]) +
+                                    qdb[o0 + r] * qd[o0 + r] + qd[o1 + r] * (qdb[o1 + r] - ed[o1 + r]));
+    }
+#undef WOF
+    *Qout = Q;
+    plan3_free(&p3);
+    free(q); free(qd); free(qb); free(qdb); free(e); free(ed);
+    free(WA); free(WB); free(tmp); free(buf);
+    return L;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Anderson mixing (addition; the reference has only simple mixing).          */
 /* Residual per field, the same one simple mixing steps along:               */
 /*   eta = (wA+wB-chiN)/2, psi = 1-phiA-phiB,                                */

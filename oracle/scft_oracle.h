@@ -96,6 +96,15 @@ double orc_iterate(const orc_params *p, const double *kxyzdz, double *wA, double
 void orc_run(const orc_params *p, double *wA, double *wB, double *phiA, double *phiB,
              int iters, double *fe_hist);
 
+/* ---- discrete adjoint of one chain evaluation (addition) ----
+   For fields uA, uB used as given (no mean subtraction), forward
+   phi = getConc(uA, uB) and L = sum_r(lamA phiA + lamB phiB) + mu ln Q;
+   returns L and gA = dL/duA, gB = dL/duB (reverse mode through the marches,
+   trapezoid sum and Q).  phiA/phiB/Q receive the forward values. */
+double orc_conc_adjoint(const orc_params *p, const double *kxyzdz, const double *uA, const double *uB,
+                        const double *lamA, const double *lamB, double mu, double *phiA, double *phiB,
+                        double *Q, double *gA, double *gB);
+
 /* ---- Anderson mixing (addition; no reference counterpart) ----
    History of m previous (w, residual) pairs; FP64 least squares. */
 typedef struct orc_anderson orc_anderson;

@@ -54,6 +54,7 @@ ABI_SYMBOLS = (
     "scft_last_error", "scft_set_fields", "scft_get_fields", "scft_propagate",
     "scft_concentrations", "scft_update_fields", "scft_free_energy", "scft_iterate",
     "scft_time_iterations", "scft_time_pass", "scft_autotune", "scft_sizes",
+    "scft_conc_adjoint", "scft_adjoint_loop",
 )
 
 
@@ -73,7 +74,7 @@ class Params(ctypes.Structure):
         ("wopt", ctypes.c_double), ("wcmp", ctypes.c_double),
         ("anderson_m", ctypes.c_int), ("anderson_mix", ctypes.c_double),
         ("pi", ctypes.c_double), ("use_graph", ctypes.c_int), ("xy_plane", ctypes.c_int),
-        ("tiling", ctypes.c_int),
+        ("tiling", ctypes.c_int), ("checkpoint", ctypes.c_int),
     ]
 
     def replace(self, **kw):
@@ -118,6 +119,9 @@ def lib():
                                             ctypes.POINTER(ctypes.c_int), _dp]),
             "scft_time_iterations": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
             "scft_time_pass": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+            "scft_conc_adjoint": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                                 _dp, _dp]),
+            "scft_adjoint_loop": (ctypes.c_int, [vp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]),
             "scft_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_float)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
@@ -243,6 +247,39 @@ class Solver:
         fe = np.empty(5 * self.batch)
         self._check(lib().scft_iterate(self._h, int(n), float(tol), ctypes.byref(done), _ptr(fe)))
         return done.value, fe.reshape(self.batch, 5)
+
+    # -- adjoint --
+    def conc_adjoint(self, uA, uB, lamA=None, lamB=None, mu=0.0):
+        """L = sum(lamA phiA + lamB phiB) + mu ln Q at fields u (used as given).
+        Returns (L[batch], gA, gB, phiA, phiB, Q[batch]), g = dL/du, arrays [batch][N]."""
+        n = self.batch * self.N
+        uA = _arr(uA, n)
+        uB = _arr(uB, n)
+        lA = None if lamA is None else _arr(lamA, n)
+        lB = None if lamB is None else _arr(lamB, n)
+        L = np.empty(self.batch)
+        Q = np.empty(self.batch)
+        out = [np.empty(n) for _ in range(4)]  # phiA, phiB, gA, gB
+        self._check(lib().scft_conc_adjoint(self._h, _ptr(uA), _ptr(uB), _ptr(lA), _ptr(lB), float(mu), _ptr(L),
+                                            _ptr(out[0]), _ptr(out[1]), _ptr(Q), _ptr(out[2]), _ptr(out[3])))
+        shp = (self.batch, self.N)
+        return L, out[2].reshape(shp), out[3].reshape(shp), out[0].reshape(shp), out[1].reshape(shp), Q
+
+    def adjoint_loop(self, h, w0A, w0B, n_iter, target, grad=True):
+        """loss = sum (phiA_last - target)^2 after n_iter chain evaluations of the
+        SCFT loop with external field h on A; returns (loss[batch], grad_h[batch][N] or None, phiA_last)."""
+        n = self.batch * self.N
+        h = _arr(h, n)
+        a = _arr(w0A, n)
+        b = _arr(w0B, n)
+        t = _arr(target, n)
+        loss = np.empty(self.batch)
+        g = np.empty(n) if grad else None
+        ph = np.empty(n)
+        self._check(lib().scft_adjoint_loop(self._h, _ptr(h), _ptr(a), _ptr(b), int(n_iter), _ptr(t), _ptr(loss),
+                                            _ptr(g), _ptr(ph)))
+        shp = (self.batch, self.N)
+        return loss, (g.reshape(shp) if grad else None), ph.reshape(shp)
 
     # -- timing --
     def time_iterations(self, n):

@@ -304,6 +304,119 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
     phiB[b * N + r] = accB * f;
 }
 
+// phi contributions of the pair (s, Ns - s) from z(s) = zs and z(Ns - s) = zu
+// (checkpoint mode: one launch per pair, accumulating into phi; ffl applied
+// per term so phi is complete after the last pair)
+template <typename T>
+__global__ void __launch_bounds__(RT) k_contour_pair(const typename cplx<T>::type *__restrict__ zs,
+                                                     const typename cplx<T>::type *__restrict__ zu, int s, int NsA,
+                                                     int Ns, long N, int batch, const double *__restrict__ ffl,
+                                                     double *__restrict__ phiA, double *__restrict__ phiB, int init)
+{
+    const int b = blockIdx.y;
+    const long r = (long)blockIdx.x * RT + threadIdx.x;
+    if (r >= N) return;
+    const long i = b * N + r;
+    const double f = ffl[b];
+    double wa, wb;
+    const int u = Ns - s;
+    const double p1 = (double)zs[i].x * (double)zu[i].y;
+    trap_w(s, NsA, Ns, wa, wb);
+    double a = wa * p1, c = wb * p1;
+    if (u != s) {
+        const double p2 = (double)zu[i].x * (double)zs[i].y;
+        trap_w(u, NsA, Ns, wa, wb);
+        a += wa * p2;
+        c += wb * p2;
+    }
+    phiA[i] = (init ? 0.0 : phiA[i]) + a * f;
+    phiB[i] = (init ? 0.0 : phiB[i]) + c * f;
+}
+
+// partial sums of lamA*phiA + lamB*phiB -> part[(b*2)*nblk + blk]
+__global__ void __launch_bounds__(RT) k_dot_lamphi(const double *__restrict__ lamA, const double *__restrict__ lamB,
+                                                   const double *__restrict__ phiA, const double *__restrict__ phiB,
+                                                   long N, int nblk, double *__restrict__ part)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.y;
+    double v[1] = {0.0};
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
+        const long i = b * N + r;
+        v[0] += lamA[i] * phiA[i] + lamB[i] * phiB[i];
+    }
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[(b * 2) * nblk + blockIdx.x] = v[0];
+}
+
+// L = sum lam.phi + mu ln Q and dL/dQ = (-sum lam.phi + mu)/Q per instance;
+// out[b*2] = L, out[b*2+1] = dL/dQ
+__global__ void __launch_bounds__(RT) k_adj_scalars(const double *__restrict__ part, int nblk,
+                                                    const double *__restrict__ Q, double mu, double *__restrict__ out)
+{
+    __shared__ double sh[2 * RT / 64 + 2];
+    const int b = blockIdx.x;
+    const double lphi = finalize_sum(part + (b * 2) * nblk, nblk, sh);
+    if (threadIdx.x == 0) {
+        out[b * 2 + 0] = lphi + mu * log(Q[b]);
+        out[b * 2 + 1] = (-lphi + mu) / Q[b];
+    }
+}
+
+// Adjoint step tau of the packed march: zb = S(zb_prev) + E(tau) (zb holds the
+// propagated part on entry, or nothing when init), then the gradient terms of
+// contour points tau and Ns - tau:
+//   E(tau) = g(tau) Re z(tau)/Q + i [g(Ns-tau) Im z(tau)/Q + (tau == 0) dL/dQ / N]
+//   g(s)   = ds (cA(s) lamA + cB(s) lamB)      (trapezoid weights cA, cB)
+//   du_b  += -ds cb(tau)    [Re zb Im z(Ns-tau) - g(tau) Re z(tau) Im z(Ns-tau)/Q]
+//   du_b  += -ds cb(Ns-tau)  Im zb Re z(Ns-tau)
+// (zb(tau) = qbar_dag(tau) + i qbar(Ns - tau); z(t) = q(t) + i qdag(Ns - t)).
+template <typename T>
+__global__ void __launch_bounds__(RT) k_adj_point(typename cplx<T>::type *__restrict__ zb,
+                                                  const typename cplx<T>::type *__restrict__ zt,
+                                                  const typename cplx<T>::type *__restrict__ zu,
+                                                  const double *__restrict__ lamA, const double *__restrict__ lamB,
+                                                  int tau, int NsA, int Ns, double ds, long N, int batch,
+                                                  const double *__restrict__ Q, const double *__restrict__ sc,
+                                                  double *__restrict__ gA, double *__restrict__ gB, int init)
+{
+    using C = typename cplx<T>::type;
+    const int b = blockIdx.y;
+    const long r = (long)blockIdx.x * RT + threadIdx.x;
+    if (r >= N) return;
+    const long i = b * N + r;
+    const int u = Ns - tau;
+    const double q = Q[b];
+    const double la = lamA[i], lb = lamB[i];
+    double at, bt, au, bu;
+    trap_w(tau, NsA, Ns, at, bt);
+    trap_w(u, NsA, Ns, au, bu);
+    const double gt = ds * (at * la + bt * lb);
+    const double gu = ds * (au * la + bu * lb);
+    const C z_t = zt[i], z_u = zu[i];
+    const double ztr = (double)z_t.x, zti = (double)z_t.y, zur = (double)z_u.x, zui = (double)z_u.y;
+    double er = gt * ztr / q;
+    double ei = gu * zti / q;
+    if (tau == 0) ei += sc[b * 2 + 1] / (double)N;
+    double zbr = er, zbi = ei;
+    if (!init) {
+        const C p = zb[i];
+        zbr += (double)p.x;
+        zbi += (double)p.y;
+    }
+    zb[i] = C{(T)zbr, (T)zbi};
+    const double t1 = zbr * zui - gt * ztr * zui / q;  // qbar_dag(tau) qdag(tau) - g q qdag / Q
+    const double t2 = zbi * zur;                       // qbar(u) q(u)
+    double da = -ds * (at * t1 + au * t2);
+    double db = -ds * (bt * t1 + bu * t2);
+    if (!init) {
+        da += gA[i];
+        db += gB[i];
+    }
+    gA[i] = da;
+    gB[i] = db;
+}
+
 // free-energy partials with the updated fields: fpart[(b*3 + j)*nblk + blk],
 // j = 0: sum chiN*phiA*phiB, 1: -sum(wA phiA + wB phiB), 2: max|1 - phiA - phiB|
 __device__ __forceinline__ void fe_partials(double fab, double fw, double inc, double *sh, double *fpart, int b,
@@ -583,6 +696,8 @@ using namespace scft;
 
 static thread_local std::string g_create_error;
 
+static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out);
+
 struct scft_ctx {
     scft_params p{};
     int device = 0;
@@ -595,6 +710,10 @@ struct scft_ctx {
     int active[3] = {0, 0, 0};
     int first_axis = -1, last_axis = -1;
     int use_plane = 0;  // x and y passes fused in one xy-plane kernel
+    int ckpt = 0;       // sqrt(Ns) checkpointing of the slice store
+    int ck_K = 1, ck_n = 1;  // checkpoint spacing and count
+    // adjoint scratch (allocated on first use): uA, uB, lamA, lamB, gA, gB, phiA, phiB
+    double *adj = nullptr;
     int pass_variant[3] = {0, 0, 0};  // tiling variant per axis (autotuned)
     int nblk = 1;
     size_t csize = 16;  // bytes per complex element
@@ -673,6 +792,7 @@ extern "C" void scft_default_params(scft_params *p)
     p->use_graph = 1;
     p->xy_plane = -1;
     p->tiling = -1;
+    p->checkpoint = 0;
 }
 
 // init_cuda.cu:326-331
@@ -873,7 +993,9 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     A((void **)&c->phiA, 8 * BN);
     A((void **)&c->phiB, 8 * BN);
     A(&c->wdz, 4 * c->tsize * BN);
-    A(&c->slots, (size_t)(c->Ns + 1) * c->csize * BN);
+    c->ckpt = p.checkpoint ? 1 : 0;
+    const long nslots = slot_count(c->Ns, c->ckpt, &c->ck_K, &c->ck_n);
+    A(&c->slots, (size_t)nslots * c->csize * BN);
     A(&c->work, c->csize * BN);
     A((void **)&c->part, sizeof(double) * 2 * c->batch * c->nblk);
     A((void **)&c->part2, sizeof(double) * 2 * c->batch * c->nblk);
@@ -967,91 +1089,201 @@ extern "C" int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double
 // ---------------------------------------------------------------------------
 static unsigned pt_grid(long n) { return (unsigned)std::min<long>(4096, std::max<long>(1, (n + RT - 1) / RT)); }
 
-// march `nsteps` packed steps from slice 0; per step either three axis
-// passes or (c->use_plane) the fused xy-plane kernel and the z pass
-// block: -1 packed chain (real part q with A then B, imaginary part qdag with
-// B then A), 0 or 1 a single block's W on both parts (sovDifFft's march)
+// Slice store.  Full mode: slot t holds z(t), t = 0..Ns.  Checkpoint mode
+// (sqrt(Ns) checkpointing): with K = ceil(sqrt(Ns+1)) only z(kK) is kept
+// (checkpoint k), plus a K-slot segment buffer D (z(kK+i) recomputed from
+// checkpoint k), a ping-pong pair P and, in both modes, the adjoint ping-pong
+// pair ZB.  Slot indices:
+//   full:  z(t) = t;                     ZB = Ns+1, Ns+2
+//   ckpt:  CK[k] = k (k < nck), D[i] = nck + i (i = 1..K-1 used),
+//          P[j] = nck + K + j, ZB[j] = nck + K + 2 + j
+static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
+{
+    int K = 1;
+    while (K * K < Ns + 1) ++K;
+    const int nck = Ns / K + 1;
+    if (K_out) *K_out = K;
+    if (nck_out) *nck_out = nck;
+    return ckpt ? (long)nck + K + 4 : (long)Ns + 3;
+}
+
+// block of the real (br) and imaginary (bi) part at packed step t (1..Ns):
+// q(t) steps with block A for t <= NsA; qdag(Ns - t) with block B for t <= NsB
+static inline void step_blocks(const scft_ctx *c, int t, int &br, int &bi)
+{
+    br = (t <= c->NsA) ? 0 : 1;
+    bi = (t <= c->NsB) ? 1 : 0;
+}
+
+// one packed step dst = S(src): three axis passes (or the fused xy-plane
+// kernel and the z pass); the real part uses block br's W, the imaginary
+// part block bi's
 template <typename T>
-static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block)
+static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, typename cplx<T>::type *dst, int br,
+                               int bi)
 {
     using C = typename cplx<T>::type;
     const size_t BN = (size_t)c->batch * c->N;
-    C *slots = (C *)c->slots;
     C *work = (C *)c->work;
     const T *W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
-    for (int t = 1; t <= nsteps; ++t) {
-        int br, bi;  // block of the real and imaginary part at step t
-        if (block < 0) {
-            br = (t <= c->NsA) ? 0 : 1;  // q(t): block A for t <= NsA
-            bi = (t <= c->NsB) ? 1 : 0;  // qdag(Ns-t): block B for t <= NsB
-        } else {
-            br = bi = block;
-        }
-        const T *wre = W + br * BN, *wim = W + bi * BN;
-        const T *pre_r = W + (2 + br) * BN, *pre_i = W + (2 + bi) * BN;
-        const C *src = slots + (size_t)(t - 1) * BN;
-        C *dst = slots + (size_t)t * BN;
-        auto args = [&](int a, bool first, bool last) {
-            PassArgs<T> pa;
-            pa.src = first ? src : work;
-            pa.dst = last ? dst : work;
-            pa.wre = wre;
-            pa.wim = wim;
-            pa.wre_post = pre_r;
-            pa.wim_post = pre_i;
-            pa.tw = (const C *)c->tw[a];
-            pa.kf = (const T *)c->kf[a];
-            pa.nlines = (long)c->batch * (c->N / c->n[a]);
-            pa.N = c->N;
-            pa.nx = c->n[0];
-            pa.ny = c->n[1];
-            pa.nz = c->n[2];
-            pa.pre = first;
-            pa.post = last;
-            return pa;
-        };
-        if (c->use_plane) {
-            const bool zact = c->active[2];
-            PassArgs<T> pa = args(0, true, !zact);
-            hipError_t e = launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], (const T *)c->kf[1],
-                                           (long)c->batch * c->n[2], c->stream);
-            if (e != hipSuccess) return e;
-            if (zact) {
-                e = launch_pass<T>(2, c->n[2], c->pass_variant[2], args(2, false, true), c->stream);
-                if (e != hipSuccess) return e;
-            }
-            continue;
-        }
-        for (int a = 0; a < 3; ++a) {
-            if (!c->active[a]) continue;
-            hipError_t e = launch_pass<T>(a, c->n[a], c->pass_variant[a], args(a, a == c->first_axis,
-                                                                             a == c->last_axis), c->stream);
-            if (e != hipSuccess) return e;
-        }
+    const T *wre = W + br * BN, *wim = W + bi * BN;
+    const T *post_r = W + (2 + br) * BN, *post_i = W + (2 + bi) * BN;
+    auto args = [&](int a, bool first, bool last) {
+        PassArgs<T> pa;
+        pa.src = first ? src : work;
+        pa.dst = last ? dst : work;
+        pa.wre = wre;
+        pa.wim = wim;
+        pa.wre_post = post_r;
+        pa.wim_post = post_i;
+        pa.tw = (const C *)c->tw[a];
+        pa.kf = (const T *)c->kf[a];
+        pa.nlines = (long)c->batch * (c->N / c->n[a]);
+        pa.N = c->N;
+        pa.nx = c->n[0];
+        pa.ny = c->n[1];
+        pa.nz = c->n[2];
+        pa.pre = first;
+        pa.post = last;
+        return pa;
+    };
+    if (c->use_plane) {
+        const bool zact = c->active[2];
+        hipError_t e = launch_plane<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], (const T *)c->kf[1],
+                                       (long)c->batch * c->n[2], c->stream);
+        if (e != hipSuccess || !zact) return e;
+        return launch_pass<T>(2, c->n[2], c->pass_variant[2], args(2, false, true), c->stream);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!c->active[a]) continue;
+        hipError_t e = launch_pass<T>(a, c->n[a], c->pass_variant[a], args(a, a == c->first_axis, a == c->last_axis),
+                                      c->stream);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-// getConc on the device; mean_part: partial sums of wA to subtract first (or null)
 template <typename T>
-static hipError_t enqueue_conc(scft_ctx *c, const double *mean_part)
+static typename cplx<T>::type *slot_ptr(scft_ctx *c, long k)
+{
+    return (typename cplx<T>::type *)c->slots + (size_t)k * c->batch * c->N;
+}
+
+// march `nsteps` steps from slot 0.  block -1: the packed chain; 0/1: one
+// block's W on both parts (sovDifFft's march, full mode only).  In checkpoint
+// mode only z(kK) and the last slice are kept; *last receives z(nsteps).
+template <typename T>
+static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block, typename cplx<T>::type **last = nullptr)
 {
     using C = typename cplx<T>::type;
-    const size_t BN = (size_t)c->batch * c->N;
+    C *cur = slot_ptr<T>(c, 0);
+    for (int t = 1; t <= nsteps; ++t) {
+        int br, bi;
+        if (block < 0) step_blocks(c, t, br, bi);
+        else br = bi = block;
+        C *dst;
+        if (!c->ckpt) dst = slot_ptr<T>(c, t);
+        else if (t % c->ck_K == 0) dst = slot_ptr<T>(c, t / c->ck_K);
+        else dst = slot_ptr<T>(c, c->ck_n + c->ck_K + (t & 1));
+        hipError_t e = enqueue_step<T>(c, cur, dst, br, bi);
+        if (e != hipSuccess) return e;
+        cur = dst;
+    }
+    if (last) *last = cur;
+    return hipSuccess;
+}
+
+// Visits tau = 0, 1, ... and yields z(tau) (ascending) and z(Ns - tau)
+// (descending).  Full mode reads the store; checkpoint mode re-marches z(tau)
+// from checkpoint 0 (checkpoints are reused when tau hits one) and recomputes
+// the K-point segment holding Ns - tau from its checkpoint when it changes --
+// about 2 Ns extra steps for a full traversal, with O(sqrt Ns) slices held.
+template <typename T>
+struct PairWalk {
+    using C = typename cplx<T>::type;
+    scft_ctx *c;
+    C *asc = nullptr;
+    int seg = -1;
+    explicit PairWalk(scft_ctx *cc) : c(cc) {}
+
+    hipError_t next(int tau, const C **zt, const C **zu)
+    {
+        const int Ns = c->Ns, K = c->ck_K, nck = c->ck_n;
+        if (!c->ckpt) {
+            *zt = slot_ptr<T>(c, tau);
+            *zu = slot_ptr<T>(c, Ns - tau);
+            return hipSuccess;
+        }
+        // ascending
+        if (tau % K == 0) {
+            asc = slot_ptr<T>(c, tau / K);
+        } else {
+            int br, bi;
+            step_blocks(c, tau, br, bi);
+            C *dst = slot_ptr<T>(c, nck + K + (tau & 1));
+            hipError_t e = enqueue_step<T>(c, asc, dst, br, bi);
+            if (e != hipSuccess) return e;
+            asc = dst;
+        }
+        *zt = asc;
+        // descending: segment k covers s = kK .. min(kK + K - 1, Ns)
+        const int s = Ns - tau, k = s / K;
+        if (k != seg) {
+            const int len = std::min(K - 1, Ns - k * K);
+            const C *prev = slot_ptr<T>(c, k);
+            for (int i = 1; i <= len; ++i) {
+                int br, bi;
+                step_blocks(c, k * K + i, br, bi);
+                C *dst = slot_ptr<T>(c, nck + i);
+                hipError_t e = enqueue_step<T>(c, prev, dst, br, bi);
+                if (e != hipSuccess) return e;
+                prev = dst;
+            }
+            seg = k;
+        }
+        const int i = s - k * K;
+        *zu = (i == 0) ? slot_ptr<T>(c, k) : slot_ptr<T>(c, nck + i);
+        return hipSuccess;
+    }
+};
+
+// getConc on the device for fields (fA, fB); mean_part: partial sums of fA to
+// subtract first (or null: fields used as given)
+template <typename T>
+static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB, const double *mean_part,
+                                      double *phiA, double *phiB)
+{
+    using C = typename cplx<T>::type;
     const dim3 g(c->nblk, c->batch);
-    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->N, c->nblk, mean_part,
-                       c->p.ds / 2, c->post, (T *)c->wdz, (C *)c->slots, c->batch);
+    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part, c->p.ds / 2, c->post,
+                       (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = enqueue_march<T>(c, c->Ns, -1)) != hipSuccess) return e;
-    const C *slotNs = (const C *)c->slots + (size_t)c->Ns * BN;
-    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, slotNs, c->N, c->nblk, c->part2);
+    C *zNs = nullptr;
+    if ((e = enqueue_march<T>(c, c->Ns, -1, &zNs)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, (const C *)zNs, c->N, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
                        c->ffl);
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
-    hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch, c->NsA, c->Ns,
-                       c->ffl, c->phiA, c->phiB);
+    if (!c->ckpt) {
+        hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch, c->NsA,
+                           c->Ns, c->ffl, phiA, phiB);
+        return hipGetLastError();
+    }
+    PairWalk<T> walk(c);
+    for (int tau = 0; 2 * tau <= c->Ns; ++tau) {
+        const C *zt, *zu;
+        if ((e = walk.next(tau, &zt, &zu)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_contour_pair<T>, gc, dim3(RT), 0, c->stream, zt, zu, tau, c->NsA, c->Ns, c->N,
+                           c->batch, (const double *)c->ffl, phiA, phiB, tau == 0 ? 1 : 0);
+    }
     return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t enqueue_conc(scft_ctx *c, const double *mean_part)
+{
+    return enqueue_conc_fields<T>(c, c->wA, c->wB, mean_part, c->phiA, c->phiB);
 }
 
 static hipError_t enqueue_conc_any(scft_ctx *c, const double *mean_part)
@@ -1146,6 +1378,7 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
     if (block != 0 && block != 1) return set_err(ctx, SCFT_E_INVALID, "block must be 0 (A) or 1 (B)");
     if (sign != 1 && sign != -1) return set_err(ctx, SCFT_E_INVALID, "sign must be +1 or -1");
     if (!q_out) return set_err(ctx, SCFT_E_INVALID, "q_out is NULL");
+    if (ctx->ckpt) return set_err(ctx, SCFT_E_STATE, "scft_propagate returns every slice: needs checkpoint = 0");
     HIPX(ctx, hipSetDevice(ctx->device));
     const int ns = block == 0 ? ctx->NsA : ctx->NsB;
     const size_t BN = (size_t)ctx->batch * ctx->N;
@@ -1411,5 +1644,381 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
         ms_out[2] = per_axis[2];
         ms_out[3] = tplane;
     }
+    return SCFT_OK;
+}
+
+// ===========================================================================
+// Discrete adjoint (no reference counterpart)
+// ===========================================================================
+// Reverse mode of one chain evaluation: for fields u = (uA, uB) used as given,
+// L = sum_r (lamA phiA + lamB phiB) + mu ln Q, dL/du.  The step operator
+// S = W K W is symmetric, so the adjoint recursions
+//   qbar(Ns) = e(Ns),  qbar(j-1) = e(j-1) + S qbar(j)
+//   qbar_dag(0) = ed(0), qbar_dag(j) = ed(j) + S qbar_dag(j-1)
+// (e, ed: explicit derivatives of L in q, qdag) are the forward packed march
+// with point sources: zb(tau) = qbar_dag(tau) + i qbar(Ns - tau).  Summing the
+// W-derivatives of every step gives
+//   dL/du_b = -ds trap_b[ qbar q + qbar_dag qdag - g q qdag / Q ],
+// whose terms at contour points tau and Ns - tau need only z(tau) and
+// z(Ns - tau) at adjoint step tau (k_adj_point).
+static int ensure_adjoint_scratch(scft_ctx *c)
+{
+    if (c->adj) return SCFT_OK;
+    const size_t BN = (size_t)c->batch * c->N;
+    int rc = dalloc(c, (void **)&c->adj, 8 * BN * sizeof(double) + 2 * sizeof(double) * c->batch);
+    return rc;
+}
+
+template <typename T>
+static hipError_t enqueue_conc_adjoint(scft_ctx *c, double mu)
+{
+    using C = typename cplx<T>::type;
+    const size_t BN = (size_t)c->batch * c->N;
+    double *uA = c->adj, *uB = uA + BN, *lA = uB + BN, *lB = lA + BN, *gA = lB + BN, *gB = gA + BN, *pA = gB + BN,
+           *pB = pA + BN, *sc = pB + BN;
+    hipError_t e = enqueue_conc_fields<T>(c, uA, uB, nullptr, pA, pB);
+    if (e != hipSuccess) return e;
+    const dim3 g(c->nblk, c->batch);
+    hipLaunchKernelGGL(k_dot_lamphi, g, dim3(RT), 0, c->stream, (const double *)lA, (const double *)lB,
+                       (const double *)pA, (const double *)pB, c->N, c->nblk, c->part2);
+    hipLaunchKernelGGL(k_adj_scalars, dim3(c->batch), dim3(RT), 0, c->stream, (const double *)c->part2, c->nblk,
+                       (const double *)c->Q, mu, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const long zb0 = c->ckpt ? (long)c->ck_n + c->ck_K + 2 : (long)c->Ns + 1;
+    const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+    PairWalk<T> walk(c);
+    for (int tau = 0; tau <= c->Ns; ++tau) {
+        const C *zt, *zu;
+        if ((e = walk.next(tau, &zt, &zu)) != hipSuccess) return e;
+        C *zb = slot_ptr<T>(c, zb0 + (tau & 1));
+        if (tau > 0) {
+            int br, bi;
+            step_blocks(c, tau, br, bi);
+            if ((e = enqueue_step<T>(c, slot_ptr<T>(c, zb0 + ((tau - 1) & 1)), zb, br, bi)) != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_adj_point<T>, gc, dim3(RT), 0, c->stream, zb, zt, zu, (const double *)lA,
+                           (const double *)lB, tau, c->NsA, c->Ns, c->p.ds, c->N, c->batch, (const double *)c->Q,
+                           (const double *)sc, gA, gB, tau == 0 ? 1 : 0);
+    }
+    return hipGetLastError();
+}
+
+// device forward of one chain evaluation on the adjoint scratch fields (uA, uB -> phiA, phiB, Q)
+static hipError_t enqueue_conc_scratch(scft_ctx *c)
+{
+    const size_t BN = (size_t)c->batch * c->N;
+    double *uA = c->adj, *uB = uA + BN, *pA = uB + 5 * BN, *pB = pA + BN;
+    return c->fp32 ? enqueue_conc_fields<float>(c, uA, uB, nullptr, pA, pB)
+                   : enqueue_conc_fields<double>(c, uA, uB, nullptr, pA, pB);
+}
+
+extern "C" int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *uB, const double *lamA,
+                                 const double *lamB, double mu, double *L, double *phiA, double *phiB, double *Q,
+                                 double *gA, double *gB)
+{
+    if (!ctx || !uA || !uB) return ctx ? set_err(ctx, SCFT_E_INVALID, "uA/uB are NULL") : SCFT_E_INVALID;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_adjoint_scratch(ctx);
+    if (rc) return rc;
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    double *d = ctx->adj;
+    HIPX(ctx, hipMemcpyAsync(d, uA, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(ctx, hipMemcpyAsync(d + BN, uB, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> zero;
+    if (!lamA || !lamB) zero.assign(BN, 0.0);
+    HIPX(ctx, hipMemcpyAsync(d + 2 * BN, lamA ? lamA : zero.data(), 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(ctx, hipMemcpyAsync(d + 3 * BN, lamB ? lamB : zero.data(), 8 * BN, hipMemcpyHostToDevice, ctx->stream));
+    HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, mu) : enqueue_conc_adjoint<double>(ctx, mu));
+    std::vector<double> sc(2 * ctx->batch), q(ctx->batch);
+    HIPX(ctx, hipMemcpyAsync(sc.data(), d + 8 * BN, 16 * ctx->batch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipMemcpyAsync(q.data(), ctx->Q, 8 * ctx->batch, hipMemcpyDeviceToHost, ctx->stream));
+    if (gA) HIPX(ctx, hipMemcpyAsync(gA, d + 4 * BN, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (gB) HIPX(ctx, hipMemcpyAsync(gB, d + 5 * BN, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (phiA) HIPX(ctx, hipMemcpyAsync(phiA, d + 6 * BN, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    if (phiB) HIPX(ctx, hipMemcpyAsync(phiB, d + 7 * BN, 8 * BN, hipMemcpyDeviceToHost, ctx->stream));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_conc = 0;
+    for (int b = 0; b < ctx->batch; ++b) {
+        if (!std::isfinite(q[b]) || !(q[b] > 0.0))
+            return set_err(ctx, SCFT_E_NONFINITE, "Q of instance " + std::to_string(b) + " is not finite and > 0");
+        if (L) L[b] = sc[2 * b];
+        if (Q) Q[b] = q[b];
+    }
+    return SCFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the unrolled SCFT loop with an external field h on the A blocks.
+// Forward, k = 0..n-1 (FP64 host vectors, device chain evaluations):
+//   wt = P w (mean removed per instance and field), u = (wtA + h, wtB),
+//   (phi, Q) = conc(u); if k < n-1: d = R(wt, phi), w = mix(wt, d, history)
+// Loss = sum_r (phiA(n-1) - target)^2 per instance.  R is the residual that
+// simple mixing steps along (d_A = chiN phiB + eta - wA - wcmp psi, ...);
+// mix is simple mixing (w = wt + wopt d) or Anderson (the library's
+// algorithm, with U and V formed directly from the differences).  The reverse
+// sweep runs the Anderson least-squares adjoint (cbar -> Vbar = U^-1 cbar,
+// Ubar = -Vbar c^T), the residual's linear adjoint, the chain adjoint on the
+// device, and P (symmetric) for the mean removal.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Gaussian elimination with partial pivoting on an n x n system (row major)
+bool host_solve(int n, std::vector<double> A, std::vector<double> &x)
+{
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double best = std::fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[i * n + k]) > best) {
+                best = std::fabs(A[i * n + k]);
+                piv = i;
+            }
+        if (best == 0.0) return false;
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[piv * n + j]);
+            std::swap(x[k], x[piv]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+            x[i] -= f * x[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double s = x[k];
+        for (int j = k + 1; j < n; ++j) s -= A[k * n + j] * x[j];
+        x[k] = s / A[k * n + k];
+    }
+    return true;
+}
+
+void center(double *w, long N)
+{
+    double s = 0.0;
+    for (long r = 0; r < N; ++r) s += w[r];
+    const double m = s / (double)N;
+    for (long r = 0; r < N; ++r) w[r] -= m;
+}
+
+}  // namespace
+
+extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const double *w0B, int n_iter,
+                                 const double *target, double *loss, double *grad_h, double *phiA_last)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (!h || !w0A || !w0B || !target || n_iter < 1)
+        return set_err(ctx, SCFT_E_INVALID, "scft_adjoint_loop: NULL input or n_iter < 1");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_adjoint_scratch(ctx);
+    if (rc) return rc;
+    const long N = ctx->N;
+    const int nb = ctx->batch;
+    const size_t BN = (size_t)nb * N;
+    const double chiN = ctx->p.chiN, wcmp = ctx->p.wcmp;
+    const bool anderson = ctx->p.mixing == SCFT_MIX_ANDERSON;
+    const int m = anderson ? ctx->p.anderson_m : 0;
+    const double lam = anderson ? ctx->p.anderson_mix : ctx->p.wopt;
+    const int K = n_iter;
+    // per-iteration state (FP64): centered fields, phi, residual
+    std::vector<std::vector<double>> wt(K, std::vector<double>(2 * BN)), phi(K, std::vector<double>(2 * BN)),
+        dd(K, std::vector<double>(2 * BN));
+    // Anderson per update k, per instance: mp, coefficients, U, solve status
+    std::vector<int> mpk(K, 0);
+    std::vector<std::vector<double>> ck(K), Uk(K);
+    std::vector<std::vector<char>> okk(K);
+    double *dev = ctx->adj;
+    auto inst = [&](std::vector<double> &v, int f, int b) { return v.data() + (size_t)f * BN + (size_t)b * N; };
+    auto run_conc = [&](const std::vector<double> &w, std::vector<double> &ph) -> int {
+        std::vector<double> u(2 * BN);
+        for (size_t i = 0; i < BN; ++i) {
+            u[i] = w[i] + h[i];
+            u[BN + i] = w[BN + i];
+        }
+        HIPX(ctx, hipMemcpyAsync(dev, u.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+        HIPX(ctx, enqueue_conc_scratch(ctx));
+        HIPX(ctx, hipMemcpyAsync(ph.data(), dev + 6 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<double> q(nb);
+        HIPX(ctx, hipMemcpyAsync(q.data(), ctx->Q, 8 * nb, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(ctx, hipStreamSynchronize(ctx->stream));
+        for (int b = 0; b < nb; ++b)
+            if (!std::isfinite(q[b]) || !(q[b] > 0.0))
+                return set_err(ctx, SCFT_E_NONFINITE, "Q not finite and > 0 in the adjoint's forward loop");
+        return SCFT_OK;
+    };
+    // ---- forward ----
+    std::vector<double> w(2 * BN);
+    std::memcpy(w.data(), w0A, 8 * BN);
+    std::memcpy(w.data() + BN, w0B, 8 * BN);
+    for (int k = 0; k < K; ++k) {
+        wt[k] = w;
+        for (int b = 0; b < nb; ++b)
+            for (int f = 0; f < 2; ++f) center(inst(wt[k], f, b), N);
+        if ((rc = run_conc(wt[k], phi[k]))) return rc;
+        if (k == K - 1) break;
+        for (size_t i = 0; i < BN; ++i) {
+            const double a = wt[k][i], c = wt[k][BN + i], pa = phi[k][i], pb = phi[k][BN + i];
+            const double eta = (a + c - chiN) / 2, psum = 1.0 - pa - pb;
+            dd[k][i] = chiN * pb + eta - a - wcmp * psum;
+            dd[k][BN + i] = chiN * pa + eta - c - wcmp * psum;
+        }
+        const int mp = anderson ? std::min(k, m) : 0;
+        mpk[k] = mp;
+        ck[k].assign((size_t)nb * std::max(mp, 1), 0.0);
+        Uk[k].assign((size_t)nb * std::max(mp * mp, 1), 0.0);
+        okk[k].assign(nb, 0);
+        for (int b = 0; b < nb; ++b) {
+            std::vector<double> c(mp, 0.0);
+            if (mp > 0) {
+                // U_ij = <d_k - d_{k-i}, d_k - d_{k-j}>, V_i = <d_k - d_{k-i}, d_k>
+                std::vector<double> U(mp * mp), V(mp);
+                for (int i = 1; i <= mp; ++i) {
+                    double v = 0.0;
+                    for (int f = 0; f < 2; ++f) {
+                        const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b);
+                        for (long r = 0; r < N; ++r) v += (d0[r] - di[r]) * d0[r];
+                    }
+                    V[i - 1] = v;
+                    for (int j = i; j <= mp; ++j) {
+                        double u = 0.0;
+                        for (int f = 0; f < 2; ++f) {
+                            const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b),
+                                         *dj = inst(dd[k - j], f, b);
+                            for (long r = 0; r < N; ++r) u += (d0[r] - di[r]) * (d0[r] - dj[r]);
+                        }
+                        U[(i - 1) * mp + (j - 1)] = U[(j - 1) * mp + (i - 1)] = u;
+                    }
+                }
+                std::vector<double> x = V;
+                const bool ok = host_solve(mp, U, x);
+                okk[k][b] = ok;
+                if (ok) c = x;
+                std::copy(U.begin(), U.end(), Uk[k].begin() + (size_t)b * mp * mp);
+                std::copy(c.begin(), c.end(), ck[k].begin() + (size_t)b * mp);
+            }
+            for (int f = 0; f < 2; ++f) {
+                const double *w0 = inst(wt[k], f, b), *d0 = inst(dd[k], f, b);
+                double *wo = w.data() + (size_t)f * BN + (size_t)b * N;
+                for (long r = 0; r < N; ++r) {
+                    double wn = w0[r], dn = d0[r];
+                    for (int i = 1; i <= mp; ++i) {
+                        wn += c[i - 1] * (inst(wt[k - i], f, b)[r] - w0[r]);
+                        dn += c[i - 1] * (inst(dd[k - i], f, b)[r] - d0[r]);
+                    }
+                    wo[r] = wn + lam * dn;
+                }
+            }
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        double L = 0.0;
+        const double *pa = inst(phi[K - 1], 0, b);
+        for (long r = 0; r < N; ++r) {
+            const double e = pa[r] - target[(size_t)b * N + r];
+            L += e * e;
+        }
+        if (loss) loss[b] = L;
+    }
+    if (phiA_last) std::memcpy(phiA_last, phi[K - 1].data(), 8 * BN);
+    if (!grad_h) return SCFT_OK;
+    // ---- reverse ----
+    std::vector<std::vector<double>> Wtb(K, std::vector<double>(2 * BN, 0.0)), Db(K, std::vector<double>(2 * BN, 0.0));
+    std::vector<double> wbar(2 * BN, 0.0), hbar(BN, 0.0), pbar(2 * BN), ubar(2 * BN);
+    for (int k = K - 1; k >= 0; --k) {
+        std::fill(pbar.begin(), pbar.end(), 0.0);
+        if (k == K - 1) {
+            for (size_t i = 0; i < BN; ++i) pbar[i] = 2.0 * (phi[k][i] - target[i]);
+        } else {
+            const int mp = mpk[k];
+            for (int b = 0; b < nb; ++b) {
+                const double *c = ck[k].data() + (size_t)b * std::max(mp, 1);
+                double sc = 0.0;
+                for (int i = 0; i < mp; ++i) sc += c[i];
+                for (int f = 0; f < 2; ++f) {
+                    const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
+                    double *wt0 = inst(Wtb[k], f, b), *d0 = inst(Db[k], f, b);
+                    for (long r = 0; r < N; ++r) {
+                        wt0[r] += (1.0 - sc) * wb[r];
+                        d0[r] += lam * (1.0 - sc) * wb[r];
+                    }
+                    for (int i = 1; i <= mp; ++i) {
+                        double *wti = inst(Wtb[k - i], f, b), *di = inst(Db[k - i], f, b);
+                        for (long r = 0; r < N; ++r) {
+                            wti[r] += c[i - 1] * wb[r];
+                            di[r] += lam * c[i - 1] * wb[r];
+                        }
+                    }
+                }
+                if (mp > 0 && okk[k][b]) {
+                    // cbar_i = <wbar, (wt_{k-i} - wt_k) + lam (d_{k-i} - d_k)>
+                    std::vector<double> cbar(mp, 0.0);
+                    for (int i = 1; i <= mp; ++i) {
+                        double v = 0.0;
+                        for (int f = 0; f < 2; ++f) {
+                            const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
+                            const double *w0 = inst(wt[k], f, b), *wi = inst(wt[k - i], f, b);
+                            const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b);
+                            for (long r = 0; r < N; ++r) v += wb[r] * ((wi[r] - w0[r]) + lam * (di[r] - d0[r]));
+                        }
+                        cbar[i - 1] = v;
+                    }
+                    std::vector<double> U(Uk[k].begin() + (size_t)b * mp * mp,
+                                          Uk[k].begin() + (size_t)(b + 1) * mp * mp);
+                    std::vector<double> Vb = cbar;  // U symmetric: Vbar = U^-1 cbar
+                    host_solve(mp, U, Vb);
+                    // Ubar_ij = -Vbar_i c_j;  Dbar_i = sum_j (Ubar_ij + Ubar_ji) Delta_j + Vbar_i d_k
+                    for (int f = 0; f < 2; ++f) {
+                        const double *d0 = inst(dd[k], f, b);
+                        double *db0 = inst(Db[k], f, b);
+                        for (long r = 0; r < N; ++r) {
+                            double acc0 = 0.0;
+                            for (int i = 1; i <= mp; ++i) {
+                                const double Di = d0[r] - inst(dd[k - i], f, b)[r];
+                                double Dbi = Vb[i - 1] * d0[r];
+                                for (int j = 1; j <= mp; ++j) {
+                                    const double Dj = d0[r] - inst(dd[k - j], f, b)[r];
+                                    Dbi += (-Vb[i - 1] * c[j - 1] - Vb[j - 1] * c[i - 1]) * Dj;
+                                }
+                                acc0 += Vb[i - 1] * Di + Dbi;
+                                inst(Db[k - i], f, b)[r] -= Dbi;
+                            }
+                            db0[r] += acc0;
+                        }
+                    }
+                }
+            }
+            // residual d_k = R(wt_k, phi_k): linear adjoint
+            for (size_t i = 0; i < BN; ++i) {
+                const double da = Db[k][i], db = Db[k][BN + i];
+                Wtb[k][i] += -0.5 * da + 0.5 * db;
+                Wtb[k][BN + i] += 0.5 * da - 0.5 * db;
+                pbar[i] += wcmp * da + (chiN + wcmp) * db;
+                pbar[BN + i] += (chiN + wcmp) * da + wcmp * db;
+            }
+        }
+        // chain adjoint on the device: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
+        {
+            std::vector<double> u(2 * BN);
+            for (size_t i = 0; i < BN; ++i) {
+                u[i] = wt[k][i] + h[i];
+                u[BN + i] = wt[k][BN + i];
+            }
+            HIPX(ctx, hipMemcpyAsync(dev, u.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+            HIPX(ctx, hipMemcpyAsync(dev + 2 * BN, pbar.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+            HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0) : enqueue_conc_adjoint<double>(ctx, 0.0));
+            HIPX(ctx, hipMemcpyAsync(ubar.data(), dev + 4 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
+            HIPX(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        for (size_t i = 0; i < BN; ++i) {
+            hbar[i] += ubar[i];
+            Wtb[k][i] += ubar[i];
+            Wtb[k][BN + i] += ubar[BN + i];
+        }
+        // mean removal P is symmetric: wbar(k-1) = P Wtbar(k)
+        wbar = Wtb[k];
+        for (int b = 0; b < nb; ++b)
+            for (int f = 0; f < 2; ++f) center(wbar.data() + (size_t)f * BN + (size_t)b * N, N);
+    }
+    std::memcpy(grad_h, hbar.data(), 8 * BN);
+    ctx->have_conc = 0;
     return SCFT_OK;
 }

@@ -1,0 +1,178 @@
+"""GPU tests of sqrt(Ns) checkpointing and the discrete adjoint.
+
+Checks: checkpointed concentrations equal the full-store ones; the chain
+adjoint equals the oracle's independent FP64 adjoint (1e-10) and central
+finite differences; dlnQ/du = -phi/N; the unrolled-loop gradient (simple and
+Anderson mixing, external field h on A) equals central finite differences of
+the loop's own forward along random directions (eps = 1e-5); the FP32-form
+gradient is stated against the FP64 form."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(pkg, n, L, chiN, fA, ds, prec=0, **kw):
+    nx, ny, nz = n if isinstance(n, tuple) else (n, n, n)
+    return pkg.default_params(nx=nx, ny=ny, nz=nz, lx=L, ly=L, lz=L, chiN=chiN, fA=fA, ds=ds, precision=prec, **kw)
+
+
+def _oparams(orc, p):
+    return orc.params(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, p.chiN, p.fA, p.ds, pi=p.pi, wopt=p.wopt, wcmp=p.wcmp)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("shape,fA,ds,prec", [((16, 16, 16), 0.35, 0.01, 0), ((12, 20, 8), 0.3, 0.02, 0),
+                                              ((32, 32, 32), 0.5, 0.01, 1), ((8, 8, 8), 0.45, 0.05, 0)])
+def test_checkpoint_conc_matches_full(pkg, shape, fA, ds, prec):
+    print("case", shape, fA, ds, prec, flush=True)
+    out = []
+    for ck in (0, 1):
+        p = _params(pkg, shape, 4.0, 16.0, fA, ds, prec=prec, checkpoint=ck)
+        wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, 4.0, 4.0, 4.0, "gyroid", 1.0, 0.2, 71)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations()[0]
+            phiA, phiB, _, _ = s.get_fields()
+            if ck:
+                assert s.device_bytes < out[0][3]  # fewer slices held
+            out.append((Q, phiA, phiB, s.device_bytes))
+    tol = 1e-13 if prec == 0 else 1e-6
+    assert abs(out[0][0] - out[1][0]) / out[0][0] < tol
+    assert _rel(out[1][1], out[0][1]) < tol and _rel(out[1][2], out[0][2]) < tol
+
+
+@pytest.mark.parametrize("ck", [0, 1])
+@pytest.mark.parametrize("mu", [0.0, 1.0])
+def test_conc_adjoint_vs_oracle(pkg, orc, ck, mu):
+    shape = (12, 10, 8)
+    p = _params(pkg, shape, 3.0, 14.0, 0.4, 0.02, checkpoint=ck)
+    N = 960
+    rng = np.random.default_rng(5)
+    uA, uB = rng.uniform(-1.5, 1.5, N), rng.uniform(-1.5, 1.5, N)
+    lA, lB = rng.standard_normal(N), rng.standard_normal(N)
+    with pkg.Solver(p) as s:
+        L, gA, gB, pa, pb, Q = s.conc_adjoint(uA, uB, lA, lB, mu)
+    op = _oparams(orc, p)
+    Lo, gAo, gBo, pao, pbo, Qo = orc.conc_adjoint(op, orc.ktable(op), uA, uB, lA, lB, mu)
+    print(f"ck={ck} mu={mu}: relL={abs(L[0]-Lo)/abs(Lo):.2e} relgA={_rel(gA[0], gAo):.2e} relgB={_rel(gB[0], gBo):.2e}")
+    assert abs(L[0] - Lo) / abs(Lo) < 1e-11
+    assert abs(Q[0] - Qo) / Qo < 1e-12
+    assert _rel(pa[0], pao) < 1e-12
+    assert _rel(gA[0], gAo) < 1e-10 and _rel(gB[0], gBo) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_dlnQ_du_is_minus_phi_over_N(pkg, prec):
+    p = _params(pkg, (16, 16, 16), 4.0, 18.0, 0.5, 0.01, prec=prec)
+    N = 4096
+    wA, wB = pkg.fields.seeded_fields(16, 16, 16, 4.0, 4.0, 4.0, "lamellar", 1.5, 0.2, 72)
+    with pkg.Solver(p) as s:
+        L, gA, gB, pa, pb, Q = s.conc_adjoint(wA, wB, None, None, 1.0)
+    tol = 1e-12 if prec == 0 else 1e-5
+    eA, eB = _rel(gA[0], -pa[0] / N), _rel(gB[0], -pb[0] / N)
+    print(f"prec={prec}: rel |dlnQ/du + phi/N| A={eA:.2e} B={eB:.2e}")
+    assert eA < tol and eB < tol
+
+
+def test_conc_adjoint_fd_gpu(pkg):
+    p = _params(pkg, (16, 16, 8), 4.0, 16.0, 0.35, 0.02)
+    N = 2048
+    rng = np.random.default_rng(6)
+    uA, uB = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    lA, lB = rng.standard_normal(N), rng.standard_normal(N)
+    eps = 1e-5
+    with pkg.Solver(p) as s:
+        L, gA, gB, *_ = s.conc_adjoint(uA, uB, lA, lB, 0.3)
+        for k in range(4):
+            vA, vB = rng.standard_normal(N), rng.standard_normal(N)
+            Lp = s.conc_adjoint(uA + eps * vA, uB + eps * vB, lA, lB, 0.3)[0][0]
+            Lm = s.conc_adjoint(uA - eps * vA, uB - eps * vB, lA, lB, 0.3)[0][0]
+            fd = (Lp - Lm) / (2 * eps)
+            ad = float(gA[0] @ vA + gB[0] @ vB)
+            assert abs(fd - ad) / abs(ad) < 1e-7, (k, fd, ad)
+
+
+def _loop_case(pkg, mixing, prec=0, n=12, iters=8):
+    p = _params(pkg, (n, n, n), 4.0, 18.0, 0.5, 0.02, prec=prec, mixing=mixing, anderson_m=4, anderson_mix=0.2)
+    N = n ** 3
+    w0A, w0B = pkg.fields.seeded_fields(n, n, n, 4.0, 4.0, 4.0, "lamellar", 0.5, 0.1, 73)
+    x = pkg.fields.coords(n, n, n, 4.0, 4.0, 4.0)[0]
+    target = 0.5 + 0.3 * np.cos(2 * np.pi * x / 4.0 + 0.4)
+    h = 0.05 * pkg.fields.splitmix_uniform(74, N)
+    return p, N, h, w0A, w0B, target, iters
+
+
+@pytest.mark.parametrize("mixing", [0, 1])
+def test_adjoint_loop_fd(pkg, mixing):
+    """Gradient of the loss through the unrolled loop vs central finite
+    differences of the same forward along random directions."""
+    p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing)
+    rng = np.random.default_rng(8)
+    eps = 1e-5
+    with pkg.Solver(p) as s:
+        loss, g, _ = s.adjoint_loop(h, w0A, w0B, iters, target)
+        worst = 0.0
+        for k in range(6):
+            v = rng.standard_normal(N)
+            lp = s.adjoint_loop(h + eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+            lm = s.adjoint_loop(h - eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+            fd = (lp - lm) / (2 * eps)
+            ad = float(g[0] @ v)
+            worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-12))
+    print(f"mixing={mixing}: loss={loss[0]:.6e} worst rel FD mismatch={worst:.2e}")
+    assert worst < 1e-6
+
+
+def test_adjoint_loop_fp32_vs_fp64(pkg):
+    res = {}
+    for prec in (0, 1):
+        p, N, h, w0A, w0B, target, iters = _loop_case(pkg, 1, prec=prec)
+        with pkg.Solver(p) as s:
+            res[prec] = s.adjoint_loop(h, w0A, w0B, iters, target)
+    dl = abs(res[1][0][0] - res[0][0][0]) / res[0][0][0]
+    dg = _rel(res[1][1][0], res[0][1][0])
+    print(f"FP32 vs FP64 adjoint: rel dloss={dl:.2e} rel dgrad={dg:.2e}")
+    assert dl < 1e-4 and dg < 1e-3
+
+
+def test_adjoint_loop_forward_matches_iterate(pkg):
+    """The adjoint's forward (host mixing) follows the solver's own loop."""
+    n = 12
+    p = _params(pkg, (n, n, n), 4.0, 18.0, 0.5, 0.02)
+    N = n ** 3
+    w0A, w0B = pkg.fields.seeded_fields(n, n, n, 4.0, 4.0, 4.0, "lamellar", 0.5, 0.1, 75)
+    with pkg.Solver(p) as s:
+        _, _, phi_last = s.adjoint_loop(np.zeros(N), w0A, w0B, 6, np.zeros(N), grad=False)
+        s.set_fields(w0A, w0B)
+        s.iterate(5)
+        # sixth chain evaluation of the solver's loop: subtract means, then getConc
+        _, _, wA, wB = s.get_fields()
+        wA = wA[0] - wA[0].mean()
+        wB = wB[0] - wB[0].mean()
+        _, _, _, pa, _, _ = s.conc_adjoint(wA, wB)
+    assert _rel(phi_last[0], pa[0]) < 1e-11
+
+
+def test_torch_autograd_wrapper(pkg):
+    """torch.autograd.gradcheck (FP64 finite differences) through the wrapper on a 4^3 melt."""
+    import importlib
+    import torch
+    tw = importlib.import_module("scft_amd.torch_adjoint")
+    n = 4
+    p = _params(pkg, (n, n, n), 2.0, 12.0, 0.5, 0.05)
+    N = n ** 3
+    w0A, w0B = pkg.fields.seeded_fields(n, n, n, 2.0, 2.0, 2.0, "lamellar", 0.5, 0.1, 76)
+    target = 0.5 + 0.2 * np.cos(2 * np.pi * pkg.fields.coords(n, n, n, 2.0, 2.0, 2.0)[0] / 2.0)
+    with pkg.Solver(p) as s:
+        h = torch.tensor(0.05 * pkg.fields.splitmix_uniform(77, N), dtype=torch.float64, requires_grad=True)
+        args = (s, torch.tensor(w0A), torch.tensor(w0B), 5, torch.tensor(target))
+        loss = tw.SCFTLoopLoss.apply(h, *args)
+        loss.backward()
+        _, g, _ = s.adjoint_loop(h.detach().numpy(), w0A, w0B, 5, target)
+        assert np.allclose(h.grad.numpy(), g[0], rtol=0, atol=0)
+        assert torch.autograd.gradcheck(lambda x: tw.SCFTLoopLoss.apply(x, *args), (h,), eps=1e-6, atol=1e-6,
+                                        rtol=1e-5)
