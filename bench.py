@@ -122,6 +122,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-128", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c3_fp32", "c3_fp64"],
                     help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
     a = ap.parse_args()
@@ -178,6 +179,26 @@ def main():
         }
         t32.close()
         t64.close()
+    if not a.no_sweep:
+        log("[bench] c4 sweep: 256 instances, grids 32/48/64, FP64")
+        inst = pkg.fields.sweep_instances(count=256, seed=4)
+        base = pkg.default_params(ds=0.01, precision=pkg.FP64)
+        sw = pkg.Sweep(base, inst)
+        sw.autotune(5)
+        for i, (wa, wb) in enumerate(pkg.fields.sweep_fields(inst, noise=0.1, seed=400)):
+            sw.set_fields(i, wa, wb)
+        sw.iterate(1)
+        ms = sw.time_iterations(3) / 3
+        pts = sum(n ** 3 for (n, L, c, f) in inst)
+        nsum = sum(s_[1] + s_[2] for s_ in sw.sizes)
+        forms["c4_sweep"] = {
+            "instances": len(inst), "grids": {str(g): sum(1 for x in inst if x[0] == g) for g in (32, 48, 64)},
+            "ms_per_sweep_iteration": ms, "instance_iters_per_s": len(inst) * 1e3 / ms,
+            "grid_point_steps_per_s": pts * (nsum / len(inst)) * 1e3 / ms,
+            "iter_hbm_frac": sum(iter_bytes(n ** 3, s_[1] + s_[2], 8) for (n, L, c, f), s_ in zip(inst, sw.sizes))
+            * 1e3 / ms / 1e9 / HBM_PEAK_GBPS,
+        }
+        sw.close()
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, host cores)")

@@ -168,6 +168,35 @@ int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const d
    used (0/1); ms (may be NULL): float[4] = best x, y, z pass and plane times. */
 int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 
+/* ---- heterogeneous parameter sweep (no reference counterpart: the
+   reference's batch shares chiN, f, box and grid, struct.h:42-68) ----
+   Independent instances with their own cubic grid n^3, box L^3, chiN and fA
+   (so their own NsA, NsB); everything else (ds, precision, mixing, pi) comes
+   from `base`.  Instances are grouped by grid size, one SCFT iteration of the
+   whole sweep is one captured HIP graph, and scft_sweep_autotune tunes each
+   group's passes.  Instance indices follow the input order. */
+typedef struct scft_sweep scft_sweep;
+typedef struct {
+    int n;       /* grid n x n x n */
+    double L;    /* box L x L x L (Rg) */
+    double chiN;
+    double fA;
+} scft_instance;
+
+int scft_sweep_create(const scft_params *base, const scft_instance *inst, int count, int device,
+                      scft_sweep **out);
+int scft_sweep_destroy(scft_sweep *sw);
+const char *scft_sweep_last_error(const scft_sweep *sw);
+/* fields of instance i (host, n^3 each); restarts the Anderson history of its group */
+int scft_sweep_set_fields(scft_sweep *sw, int i, const double *wA, const double *wB);
+int scft_sweep_get_fields(scft_sweep *sw, int i, double *phiA, double *phiB, double *wA, double *wB);
+/* n_iter iterations of every instance; fe (may be NULL): [count][5] as scft_iterate */
+int scft_sweep_iterate(scft_sweep *sw, int n_iter, double *fe);
+int scft_sweep_time_iterations(scft_sweep *sw, int n_iter, float *ms);
+int scft_sweep_autotune(scft_sweep *sw, int reps);
+/* N, NsA, NsB of instance i */
+int scft_sweep_size(const scft_sweep *sw, int i, long *N, int *NsA, int *NsB);
+
 /* Problem sizes: N points, NsA, NsB, device bytes allocated. */
 int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes);
 

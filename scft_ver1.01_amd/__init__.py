@@ -55,6 +55,9 @@ ABI_SYMBOLS = (
     "scft_concentrations", "scft_update_fields", "scft_free_energy", "scft_iterate",
     "scft_time_iterations", "scft_time_pass", "scft_autotune", "scft_sizes",
     "scft_conc_adjoint", "scft_adjoint_loop",
+    "scft_sweep_create", "scft_sweep_destroy", "scft_sweep_last_error", "scft_sweep_set_fields",
+    "scft_sweep_get_fields", "scft_sweep_iterate", "scft_sweep_time_iterations", "scft_sweep_autotune",
+    "scft_sweep_size",
 )
 
 
@@ -86,6 +89,11 @@ class Params(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Instance(ctypes.Structure):
+    """scft_instance of include/scft.h"""
+    _fields_ = [("n", ctypes.c_int), ("L", ctypes.c_double), ("chiN", ctypes.c_double), ("fA", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -122,6 +130,17 @@ def lib():
             "scft_conc_adjoint": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp,
                                                  _dp, _dp]),
             "scft_adjoint_loop": (ctypes.c_int, [vp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+            "scft_sweep_create": (ctypes.c_int, [P, ctypes.POINTER(Instance), ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(vp)]),
+            "scft_sweep_destroy": (ctypes.c_int, [vp]),
+            "scft_sweep_last_error": (ctypes.c_char_p, [vp]),
+            "scft_sweep_set_fields": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp]),
+            "scft_sweep_get_fields": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+            "scft_sweep_iterate": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+            "scft_sweep_time_iterations": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+            "scft_sweep_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
+            "scft_sweep_size": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long),
+                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             "scft_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_float)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
@@ -299,3 +318,67 @@ class Solver:
         ms = ctypes.c_float()
         self._check(lib().scft_time_pass(self._h, int(axis), int(reps), ctypes.byref(ms)))
         return ms.value
+
+
+class Sweep:
+    """Independent instances with their own grid, box, chiN and fA (scft_sweep_*)."""
+
+    def __init__(self, base, instances, device=0):
+        arr = (Instance * len(instances))(*[Instance(int(n), float(L), float(c), float(f))
+                                             for (n, L, c, f) in instances])
+        self._h = ctypes.c_void_p()
+        rc = lib().scft_sweep_create(ctypes.byref(base), arr, len(instances), device, ctypes.byref(self._h))
+        if rc:
+            raise ScftError(rc, lib().scft_sweep_last_error(None).decode())
+        self.count = len(instances)
+        self.sizes = []
+        for i in range(self.count):
+            N = ctypes.c_long()
+            a = ctypes.c_int()
+            b = ctypes.c_int()
+            lib().scft_sweep_size(self._h, i, ctypes.byref(N), ctypes.byref(a), ctypes.byref(b))
+            self.sizes.append((N.value, a.value, b.value))
+
+    def close(self):
+        if self._h:
+            lib().scft_sweep_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise ScftError(rc, lib().scft_sweep_last_error(self._h).decode())
+
+    def set_fields(self, i, wA, wB):
+        N = self.sizes[i][0]
+        self._check(lib().scft_sweep_set_fields(self._h, i, _ptr(_arr(wA, N)), _ptr(_arr(wB, N))))
+
+    def get_fields(self, i):
+        N = self.sizes[i][0]
+        out = [np.empty(N) for _ in range(4)]
+        self._check(lib().scft_sweep_get_fields(self._h, i, *[_ptr(o) for o in out]))
+        return tuple(out)
+
+    def iterate(self, n):
+        fe = np.empty(5 * self.count)
+        self._check(lib().scft_sweep_iterate(self._h, int(n), _ptr(fe)))
+        return fe.reshape(self.count, 5)
+
+    def time_iterations(self, n):
+        ms = ctypes.c_float()
+        self._check(lib().scft_sweep_time_iterations(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps=10):
+        self._check(lib().scft_sweep_autotune(self._h, int(reps)))

@@ -23,6 +23,13 @@ enum Axis { AX_X = 0, AX_Y = 1, AX_Z = 2 };
 // pre:  the loaded value is multiplied by (wre, wim) part-wise first,
 // post: the result is multiplied by (wre_post, wim_post) part-wise before the
 //       store (these carry the 1/N of the inverse transforms).
+// Per-instance mode (step > 0): the W factors of each line's instance are
+// chosen in the kernel from nsab[instance] = (NsA, NsB) at contour step
+// `step` (real part: block A while step <= NsA; imaginary part: block B while
+// step <= NsB) out of W = [pre A, pre B, post A, post B] ([4][batch][N]), and
+// lines of instances whose chain is shorter than `step` are left untouched.
+// step == 0 uses wre..wim_post as given for every line.  kf of instance b is
+// kf + b * kf_stride.
 template <typename T>
 struct PassArgs {
     using C = typename cplx<T>::type;
@@ -32,12 +39,17 @@ struct PassArgs {
     const T *wim;
     const T *wre_post;
     const T *wim_post;
+    const T *W;          // per-instance mode: the four W arrays
+    const int2 *nsab;    // per-instance mode: (NsA, NsB) per instance
     const C *tw;   // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
-    const T *kf;   // n entries
+    const T *kf;   // n entries per instance (kf_stride apart, 0 = shared)
+    long kf_stride;
     long nlines;   // lines in this launch (batch * N / n)
     long N;        // points per instance
+    long BN;       // batch * N
     int nx, ny, nz;
     int pre, post;
+    int step;      // contour step (per-instance mode) or 0
 };
 
 // true if the pass kernels have an instantiation for line length n

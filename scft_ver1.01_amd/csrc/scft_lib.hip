@@ -152,14 +152,15 @@ __global__ void __launch_bounds__(RT) k_center(double *__restrict__ f0, double *
 // k = 0 mode of every pass is multiplied by exactly 1.
 template <typename T>
 __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const double *__restrict__ wB, long N, int nblk,
-                                             const double *__restrict__ part, double ds2, double post,
-                                             T *__restrict__ wdz, typename cplx<T>::type *__restrict__ slot0,
-                                             int batch)
+                                             const double *__restrict__ part, double ds2,
+                                             const double *__restrict__ post_b, T *__restrict__ wdz,
+                                             typename cplx<T>::type *__restrict__ slot0, int batch)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     using C = typename cplx<T>::type;
     const int b = blockIdx.y;
     const double s0 = part ? finalize_sum(part + (b * 2) * nblk, nblk, sh) : 0.0;
+    const double post = post_b[b];
     const long BN = (long)batch * N;
     for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
         const long i = b * N + r;
@@ -179,13 +180,14 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
 
 // W (pre) and W*post of one block only (sovDifFft computes W from the field as given)
 template <typename T>
-__global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, double ds2, double post,
-                                                T *__restrict__ wpre, T *__restrict__ wpost)
+__global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, long N, double ds2,
+                                                const double *__restrict__ post_b, T *__restrict__ wpre,
+                                                T *__restrict__ wpost)
 {
     for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) {
         const double e = exp(-w[i] * ds2);
         wpre[i] = (T)e;
-        wpost[i] = (T)(e * post);
+        wpost[i] = (T)(e * post_b[i / N]);
     }
 }
 
@@ -208,16 +210,21 @@ __global__ void __launch_bounds__(RT) k_take_real(const typename cplx<T>::type *
         out[i] = (double)slot[i].x;
 }
 
-// partial sums of Re z(Ns) = q(Ns)   (cal_ql, cuda_aid.cu:119-168)
+// partial sums of Re z(Ns) = q(Ns)   (cal_ql, cuda_aid.cu:119-168).  With
+// nsab (per-instance chains) instance b reads slice Ns_b of the store at
+// slots; otherwise the slice at slotNs.
 template <typename T>
-__global__ void __launch_bounds__(RT) k_qsum(const typename cplx<T>::type *__restrict__ slotNs, long N, int nblk,
+__global__ void __launch_bounds__(RT) k_qsum(const typename cplx<T>::type *__restrict__ slotNs,
+                                             const typename cplx<T>::type *__restrict__ slots,
+                                             const int2 *__restrict__ nsab, long N, int batch, int nblk,
                                              double *__restrict__ part)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     const int b = blockIdx.y;
+    const typename cplx<T>::type *z = slotNs;
+    if (nsab) z = slots + (long)(nsab[b].x + nsab[b].y) * batch * N;
     double v[1] = {0.0};
-    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT)
-        v[0] += (double)slotNs[b * N + r].x;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) v[0] += (double)z[b * N + r].x;
     block_sums<1>(v, sh);
     if (threadIdx.x == 0) part[(b * 2) * nblk + blockIdx.x] = v[0];
 }
@@ -249,11 +256,12 @@ __device__ __forceinline__ void trap_w(int s, int NsA, int Ns, double &wa, doubl
 // pairs in flight.
 template <typename T>
 __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__restrict__ slots, long N, int batch,
-                                                int NsA, int Ns, const double *__restrict__ ffl,
+                                                const int2 *__restrict__ nsab, const double *__restrict__ ffl,
                                                 double *__restrict__ phiA, double *__restrict__ phiB)
 {
     using C = typename cplx<T>::type;
     const int b = blockIdx.y;
+    const int NsA = nsab[b].x, Ns = nsab[b].x + nsab[b].y;
     const long r = (long)blockIdx.x * RT + threadIdx.x;
     if (r >= N) return;
     const long BN = (long)batch * N;
@@ -435,11 +443,12 @@ __device__ __forceinline__ void fe_partials(double fab, double fw, double inc, d
 // phi_w (cuda_aid.cu:382-416) + F partials (cuda_scft.cu:202-210)
 __global__ void __launch_bounds__(RT) k_mix_simple(double *__restrict__ wA, double *__restrict__ wB,
                                                    const double *__restrict__ phiA, const double *__restrict__ phiB,
-                                                   long N, int nblk, double hAB, double wopt, double wcmp,
-                                                   double *__restrict__ fpart)
+                                                   long N, int nblk, const double *__restrict__ chiN, double wopt,
+                                                   double wcmp, double *__restrict__ fpart)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     const int b = blockIdx.y;
+    const double hAB = chiN[b];
     double fab = 0.0, fw = 0.0, inc = 0.0;
     for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
         const long i = b * N + r;
@@ -522,12 +531,13 @@ __device__ __forceinline__ AndRing and_ring(const long *cnt, int m)
 template <int MAXM>
 __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA, const double *__restrict__ wB,
                                                   const double *__restrict__ phiA, const double *__restrict__ phiB,
-                                                  long N, int batch, int nblk, double hAB, double wcmp,
-                                                  double *__restrict__ wh, double *__restrict__ dh, const long *cnt,
-                                                  int m, double *__restrict__ part)
+                                                  long N, int batch, int nblk, const double *__restrict__ chiN,
+                                                  double wcmp, double *__restrict__ wh, double *__restrict__ dh,
+                                                  const long *cnt, int m, double *__restrict__ part)
 {
     __shared__ double sh[(MAXM + 2) * RT / 64 + MAXM + 2 + 2];
     const int b = blockIdx.y;
+    const double hAB = chiN[b];
     const AndRing rg = and_ring(cnt, m);
     const long slot = 2 * (long)batch * N;  // one ring slot: [batch][2][N]
     double acc[MAXM + 2];
@@ -641,13 +651,14 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
 __global__ void __launch_bounds__(RT) k_and_update(double *__restrict__ wA, double *__restrict__ wB,
                                                    const double *__restrict__ phiA, const double *__restrict__ phiB,
                                                    long N, int batch, int nblk, int m, const long *cnt, double mix,
-                                                   double hAB, const double *__restrict__ wh,
+                                                   const double *__restrict__ chiN, const double *__restrict__ wh,
                                                    const double *__restrict__ dh, const double *__restrict__ coef,
                                                    double *__restrict__ fpart)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     __shared__ double cs[AND_MAX];
     const int b = blockIdx.y;
+    const double hAB = chiN[b];
     const AndRing rg = and_ring(cnt, m);
     const int mp = rg.mp;
     if (threadIdx.x < AND_MAX) cs[threadIdx.x] = coef[b * AND_MAX + threadIdx.x];
@@ -714,6 +725,15 @@ struct scft_ctx {
     int ck_K = 1, ck_n = 1;  // checkpoint spacing and count
     // adjoint scratch (allocated on first use): uA, uB, lamA, lamB, gA, gB, phiA, phiB
     double *adj = nullptr;
+    // per-instance chain parameters (equal entries for a uniform context)
+    int2 *nsab = nullptr;      // device (NsA, NsB) per instance
+    double *chiN_b = nullptr;  // device chiN per instance
+    bool kf_per_inst = false;  // one k table per instance (per-instance box) or one shared
+    double *post_b = nullptr;  // device per-instance post-step factor (1/N x length-1 axis factors)
+    int Ns_max = 0;
+    std::vector<int2> nsab_h;  // host copy of nsab
+    bool blocks_uniform = true;  // every instance has the same (NsA, NsB)
+    bool own_stream = true;
     int pass_variant[3] = {0, 0, 0};  // tiling variant per axis (autotuned)
     int nblk = 1;
     size_t csize = 16;  // bytes per complex element
@@ -820,7 +840,7 @@ static void free_all(scft_ctx *c)
     c->allocs.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     c->ev0 = c->ev1 = nullptr;
     c->stream = nullptr;
 }
@@ -845,41 +865,57 @@ static void ksq_axis(int n, double l, double pi, std::vector<double> &ksq)
     for (int i = 0; i < n; i++) ksq[i] *= ksq[i];
 }
 
-static int upload_tables(scft_ctx *c)
+// k-factor and twiddle tables.  inst_L (per instance, may be null) replaces
+// the box lengths of instance b by (L_b, L_b, L_b); then every instance gets
+// its own k table (kf_stride = n) and its own post factor.
+static int upload_tables(scft_ctx *c, const double *inst_L)
 {
     const double pi = c->p.pi;
-    const double ls[3] = {c->p.lx, c->p.ly, c->p.lz};
-    std::vector<double> k2[3];
-    for (int a = 0; a < 3; ++a) ksq_axis(c->n[a], ls[a], pi, k2[a]);
-    // axes of length 1 are not transformed; their single factor exp(-ds k^2[0])
-    // (nonzero: the reference's table gives k = -2 pi/l there) and the inverse
-    // transforms' 1/N go into the post-step factor, so kf[0] = 1 exactly
-    double fold = 1.0;
-    for (int a = 0; a < 3; ++a)
-        if (!c->active[a]) fold *= std::exp(-c->p.ds * k2[a][0]);
-    c->post = fold / (double)c->N;
+    const int nb = inst_L ? c->batch : 1;
+    std::vector<double> post(c->batch);
+    std::vector<std::vector<double>> kfa(3);
+    for (int b = 0; b < nb; ++b) {
+        const double ls[3] = {inst_L ? inst_L[b] : c->p.lx, inst_L ? inst_L[b] : c->p.ly, inst_L ? inst_L[b] : c->p.lz};
+        std::vector<double> k2[3];
+        for (int a = 0; a < 3; ++a) ksq_axis(c->n[a], ls[a], pi, k2[a]);
+        // axes of length 1 are not transformed; their single factor exp(-ds k^2[0])
+        // (nonzero: the reference's table gives k = -2 pi/l there) and the inverse
+        // transforms' 1/N go into the post-step factor, so kf[0] = 1 exactly
+        double fold = 1.0;
+        for (int a = 0; a < 3; ++a)
+            if (!c->active[a]) fold *= std::exp(-c->p.ds * k2[a][0]);
+        if (inst_L) post[b] = fold / (double)c->N;
+        else std::fill(post.begin(), post.end(), fold / (double)c->N);
+        for (int a = 0; a < 3; ++a)
+            if (c->active[a])
+                for (int k = 0; k < c->n[a]; ++k) kfa[a].push_back(std::exp(-c->p.ds * k2[a][k]));
+    }
+    c->post = post[0];
+    c->kf_per_inst = inst_L != nullptr;
+    int rc;
+    if ((rc = dalloc(c, (void **)&c->post_b, sizeof(double) * c->batch))) return rc;
+    HIPX(c, hipMemcpy(c->post_b, post.data(), sizeof(double) * c->batch, hipMemcpyHostToDevice));
     for (int a = 0; a < 3; ++a) {
         if (!c->active[a]) continue;
         const int n = c->n[a];
-        std::vector<double> kf(n), twr(n), twi(n);
+        const size_t nk = kfa[a].size();  // n or batch * n
+        std::vector<double> twr(n), twi(n);
         for (int k = 0; k < n; ++k) {
-            kf[k] = std::exp(-c->p.ds * k2[a][k]);
             // exact-angle twiddles, long double then rounded
             const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
             twr[k] = (double)cosl(ang);
             twi[k] = (double)-sinl(ang);
         }
-        int rc;
         if (c->fp32) {
-            std::vector<float> kff(n), tw(2 * n);
+            std::vector<float> kff(nk), tw(2 * n);
+            for (size_t k = 0; k < nk; ++k) kff[k] = (float)kfa[a][k];
             for (int k = 0; k < n; ++k) {
-                kff[k] = (float)kf[k];
                 tw[2 * k] = (float)twr[k];
                 tw[2 * k + 1] = (float)twi[k];
             }
-            if ((rc = dalloc(c, &c->kf[a], sizeof(float) * n))) return rc;
+            if ((rc = dalloc(c, &c->kf[a], sizeof(float) * nk))) return rc;
             if ((rc = dalloc(c, &c->tw[a], sizeof(float) * 2 * n))) return rc;
-            HIPX(c, hipMemcpy(c->kf[a], kff.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(c->kf[a], kff.data(), sizeof(float) * nk, hipMemcpyHostToDevice));
             HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
         } else {
             std::vector<double> tw(2 * n);
@@ -887,16 +923,20 @@ static int upload_tables(scft_ctx *c)
                 tw[2 * k] = twr[k];
                 tw[2 * k + 1] = twi[k];
             }
-            if ((rc = dalloc(c, &c->kf[a], sizeof(double) * n))) return rc;
+            if ((rc = dalloc(c, &c->kf[a], sizeof(double) * nk))) return rc;
             if ((rc = dalloc(c, &c->tw[a], sizeof(double) * 2 * n))) return rc;
-            HIPX(c, hipMemcpy(c->kf[a], kf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(c->kf[a], kfa[a].data(), sizeof(double) * nk, hipMemcpyHostToDevice));
             HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
         }
     }
     return SCFT_OK;
 }
 
-extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
+// Context constructor.  inst_L / inst_chiN / inst_fA (length p.batch, each may
+// be null) give per-instance box lengths (cubic L, L, L), chiN and fA in place
+// of the uniform ones in p; stream, when not null, is shared (not owned).
+static int create_ctx(const scft_params *pin, const double *inst_L, const double *inst_chiN, const double *inst_fA,
+                      int device, hipStream_t stream, scft_ctx **out)
 {
     if (!out) return set_err(nullptr, SCFT_E_INVALID, "scft_create: out is NULL");
     *out = nullptr;
@@ -920,6 +960,40 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     scft_contour_counts(p.fA, p.ds, &NsA, &NsB);
     if (NsA < 1 || NsB < 1)
         return set_err(nullptr, SCFT_E_INVALID, "both blocks need at least one contour step (fA/ds, (1-fA)/ds >= 1)");
+    std::vector<int2> nsab(p.batch, int2{NsA, NsB});
+    std::vector<double> chis(p.batch, p.chiN);
+    int Ns_max = NsA + NsB;
+    bool ns_uniform = true;
+    for (int b = 0; b < p.batch; ++b) {
+        if (inst_fA) {
+            if (!(inst_fA[b] > 0.0 && inst_fA[b] < 1.0))
+                return set_err(nullptr, SCFT_E_INVALID, "instance fA must be in (0, 1)");
+            int a1, b1;
+            scft_contour_counts(inst_fA[b], p.ds, &a1, &b1);
+            if (a1 < 1 || b1 < 1)
+                return set_err(nullptr, SCFT_E_INVALID, "instance " + std::to_string(b) + ": a block has no contour step");
+            nsab[b] = int2{a1, b1};
+            Ns_max = std::max(Ns_max, a1 + b1);
+            ns_uniform = ns_uniform && (a1 + b1 == NsA + NsB);
+        }
+        if (inst_chiN) {
+            if (!std::isfinite(inst_chiN[b])) return set_err(nullptr, SCFT_E_INVALID, "instance chiN must be finite");
+            chis[b] = inst_chiN[b];
+        }
+        if (inst_L && !(inst_L[b] > 0.0 && std::isfinite(inst_L[b])))
+            return set_err(nullptr, SCFT_E_INVALID, "instance box length must be finite and > 0");
+    }
+    if (inst_fA) {
+        // the context-level counts describe instance 0
+        NsA = nsab[0].x;
+        NsB = nsab[0].y;
+    }
+    bool blocks_uniform = true;
+    for (int b = 0; b < p.batch; ++b)
+        blocks_uniform = blocks_uniform && nsab[b].x == nsab[0].x && nsab[b].y == nsab[0].y;
+    if (p.checkpoint && !blocks_uniform)
+        return set_err(nullptr, SCFT_E_UNSUPPORTED, "checkpointing needs the same NsA and NsB for every instance");
+    (void)ns_uniform;
     const int n[3] = {p.nx, p.ny, p.nz};
     for (int a = 0; a < 3; ++a)
         if (n[a] > 1 && (n[a] & 1))
@@ -940,6 +1014,8 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     c->NsA = NsA;
     c->NsB = NsB;
     c->Ns = NsA + NsB;
+    c->Ns_max = Ns_max;
+    c->blocks_uniform = blocks_uniform;
     c->N = (long)p.nx * p.ny * p.nz;
     c->batch = p.batch;
     c->fp32 = p.precision == SCFT_FP32;
@@ -974,8 +1050,14 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     {
         hipError_t e = hipSetDevice(device);
         if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        if (stream) {
+            c->stream = stream;
+            c->own_stream = false;
+        } else {
+            e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+            if (e != hipSuccess)
+                return set_err(nullptr, SCFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
         e = hipEventCreate(&c->ev0);
         if (e == hipSuccess) e = hipEventCreate(&c->ev1);
         if (e != hipSuccess) {
@@ -994,7 +1076,9 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     A((void **)&c->phiB, 8 * BN);
     A(&c->wdz, 4 * c->tsize * BN);
     c->ckpt = p.checkpoint ? 1 : 0;
-    const long nslots = slot_count(c->Ns, c->ckpt, &c->ck_K, &c->ck_n);
+    const long nslots = slot_count(c->Ns_max, c->ckpt, &c->ck_K, &c->ck_n);
+    A((void **)&c->nsab, sizeof(int2) * c->batch);
+    A((void **)&c->chiN_b, sizeof(double) * c->batch);
     A(&c->slots, (size_t)nslots * c->csize * BN);
     A(&c->work, c->csize * BN);
     A((void **)&c->part, sizeof(double) * 2 * c->batch * c->nblk);
@@ -1019,10 +1103,19 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
         free_all(c.get());
         return set_err(nullptr, rc, m);
     }
-    if ((rc = upload_tables(c.get())) != SCFT_OK) {
+    if ((rc = upload_tables(c.get(), inst_L)) != SCFT_OK) {
         std::string m = c->err;
         free_all(c.get());
         return set_err(nullptr, rc, m);
+    }
+    {
+        hipError_t e = hipMemcpy(c->nsab, nsab.data(), sizeof(int2) * c->batch, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->chiN_b, chis.data(), sizeof(double) * c->batch, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_all(c.get());
+            return set_err(nullptr, SCFT_E_HIP, std::string("parameter upload: ") + hipGetErrorString(e));
+        }
+        c->nsab_h = nsab;
     }
     {
         hipError_t e = hipMemsetAsync(c->wA, 0, 8 * BN, c->stream);
@@ -1038,6 +1131,11 @@ extern "C" int scft_create(const scft_params *pin, int device, scft_ctx **out)
     }
     *out = c.release();
     return SCFT_OK;
+}
+
+extern "C" int scft_create(const scft_params *p, int device, scft_ctx **out)
+{
+    return create_ctx(p, nullptr, nullptr, nullptr, device, nullptr, out);
 }
 
 extern "C" int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes)
@@ -1116,11 +1214,13 @@ static inline void step_blocks(const scft_ctx *c, int t, int &br, int &bi)
 }
 
 // one packed step dst = S(src): three axis passes (or the fused xy-plane
-// kernel and the z pass); the real part uses block br's W, the imaginary
-// part block bi's
+// kernel and the z pass).  t > 0: contour step t of every instance's packed
+// chain (the kernels pick each instance's blocks, and leave instances whose
+// chain is shorter than t alone); t == 0: the real part uses block br's W and
+// the imaginary part block bi's for every instance.
 template <typename T>
-static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, typename cplx<T>::type *dst, int br,
-                               int bi)
+static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, typename cplx<T>::type *dst, int t,
+                               int br, int bi)
 {
     using C = typename cplx<T>::type;
     const size_t BN = (size_t)c->batch * c->N;
@@ -1136,8 +1236,13 @@ static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, t
         pa.wim = wim;
         pa.wre_post = post_r;
         pa.wim_post = post_i;
+        pa.W = W;
+        pa.nsab = (const int2 *)c->nsab;
+        pa.step = t;
+        pa.BN = (long)BN;
         pa.tw = (const C *)c->tw[a];
         pa.kf = (const T *)c->kf[a];
+        pa.kf_stride = c->kf_per_inst ? c->n[a] : 0;
         pa.nlines = (long)c->batch * (c->N / c->n[a]);
         pa.N = c->N;
         pa.nx = c->n[0];
@@ -1178,14 +1283,11 @@ static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block, typename cpl
     using C = typename cplx<T>::type;
     C *cur = slot_ptr<T>(c, 0);
     for (int t = 1; t <= nsteps; ++t) {
-        int br, bi;
-        if (block < 0) step_blocks(c, t, br, bi);
-        else br = bi = block;
         C *dst;
         if (!c->ckpt) dst = slot_ptr<T>(c, t);
         else if (t % c->ck_K == 0) dst = slot_ptr<T>(c, t / c->ck_K);
         else dst = slot_ptr<T>(c, c->ck_n + c->ck_K + (t & 1));
-        hipError_t e = enqueue_step<T>(c, cur, dst, br, bi);
+        hipError_t e = block < 0 ? enqueue_step<T>(c, cur, dst, t, 0, 0) : enqueue_step<T>(c, cur, dst, 0, block, block);
         if (e != hipSuccess) return e;
         cur = dst;
     }
@@ -1218,10 +1320,8 @@ struct PairWalk {
         if (tau % K == 0) {
             asc = slot_ptr<T>(c, tau / K);
         } else {
-            int br, bi;
-            step_blocks(c, tau, br, bi);
             C *dst = slot_ptr<T>(c, nck + K + (tau & 1));
-            hipError_t e = enqueue_step<T>(c, asc, dst, br, bi);
+            hipError_t e = enqueue_step<T>(c, asc, dst, tau, 0, 0);
             if (e != hipSuccess) return e;
             asc = dst;
         }
@@ -1232,10 +1332,8 @@ struct PairWalk {
             const int len = std::min(K - 1, Ns - k * K);
             const C *prev = slot_ptr<T>(c, k);
             for (int i = 1; i <= len; ++i) {
-                int br, bi;
-                step_blocks(c, k * K + i, br, bi);
                 C *dst = slot_ptr<T>(c, nck + i);
-                hipError_t e = enqueue_step<T>(c, prev, dst, br, bi);
+                hipError_t e = enqueue_step<T>(c, prev, dst, k * K + i, 0, 0);
                 if (e != hipSuccess) return e;
                 prev = dst;
             }
@@ -1255,19 +1353,21 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
 {
     using C = typename cplx<T>::type;
     const dim3 g(c->nblk, c->batch);
-    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part, c->p.ds / 2, c->post,
-                       (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part, c->p.ds / 2,
+                       (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     C *zNs = nullptr;
-    if ((e = enqueue_march<T>(c, c->Ns, -1, &zNs)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, (const C *)zNs, c->N, c->nblk, c->part2);
+    if ((e = enqueue_march<T>(c, c->Ns_max, -1, &zNs)) != hipSuccess) return e;
+    // full store: each instance's q(Ns_b) is slice Ns_b; checkpoint mode has one Ns
+    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, (const C *)zNs, (const C *)c->slots,
+                       c->ckpt ? (const int2 *)nullptr : (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
                        c->ffl);
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     if (!c->ckpt) {
-        hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch, c->NsA,
-                           c->Ns, c->ffl, phiA, phiB);
+        hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
         return hipGetLastError();
     }
     PairWalk<T> walk(c);
@@ -1296,22 +1396,26 @@ static hipError_t enqueue_update(scft_ctx *c)
     const dim3 g(c->nblk, c->batch);
     if (c->p.mixing == SCFT_MIX_SIMPLE) {
         hipLaunchKernelGGL(k_mix_simple, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->nblk,
-                           c->p.chiN, c->p.wopt, c->p.wcmp, c->fpart);
+                           (const double *)c->chiN_b, c->p.wopt, c->p.wcmp, c->fpart);
     } else {
         const int m = c->p.anderson_m;
         if (c->and_maxm == 8)
             hipLaunchKernelGGL(k_and_resid<8>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->apart);
         else if (c->and_maxm == 16)
             hipLaunchKernelGGL(k_and_resid<16>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->apart);
         else
             hipLaunchKernelGGL(k_and_resid<32>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, c->p.chiN, c->p.wcmp, c->wh, c->dh, c->and_cnt, m, c->apart);
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->apart);
         hipLaunchKernelGGL(k_and_solve, dim3(c->batch), dim3(RT), 0, c->stream, c->apart, c->nblk, c->and_maxm, m,
                            c->and_cnt, c->G, c->coef, c->aerr);
         hipLaunchKernelGGL(k_and_update, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch,
-                           c->nblk, m, c->and_cnt, c->p.anderson_mix, c->p.chiN, c->wh, c->dh, c->coef, c->fpart);
+                           c->nblk, m, c->and_cnt, c->p.anderson_mix, (const double *)c->chiN_b, c->wh, c->dh,
+                           c->coef, c->fpart);
     }
     hipLaunchKernelGGL(k_fe_final, dim3(c->batch), dim3(RT), 0, c->stream, c->fpart, c->nblk, c->Q, c->N, c->fe,
                        c->p.mixing == SCFT_MIX_ANDERSON ? c->and_cnt : nullptr);
@@ -1387,11 +1491,11 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
     char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->tsize;
     char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->tsize;
     if (ctx->fp32)
-        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
-                           ctx->p.ds / 2, ctx->post, (float *)wdz_b, (float *)wpost_b);
+        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN, ctx->N,
+                           ctx->p.ds / 2, (const double *)ctx->post_b, (float *)wdz_b, (float *)wpost_b);
     else
-        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
-                           ctx->p.ds / 2, ctx->post, (double *)wdz_b, (double *)wpost_b);
+        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN, ctx->N,
+                           ctx->p.ds / 2, (const double *)ctx->post_b, (double *)wdz_b, (double *)wpost_b);
     if (q_init) HIPX(ctx, hipMemcpyAsync(stage, q_init, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
     if (ctx->fp32)
         hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
@@ -1530,7 +1634,12 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.wim = (const T *)c->wdz;
     pa.wre_post = (const T *)c->wdz;
     pa.wim_post = (const T *)c->wdz;
+    pa.W = (const T *)c->wdz;
+    pa.nsab = (const int2 *)c->nsab;
+    pa.step = 0;
+    pa.BN = (long)c->batch * c->N;
     const int ax = axis == 3 ? 0 : axis;
+    pa.kf_stride = c->kf_per_inst ? c->n[ax] : 0;
     pa.tw = (const C *)c->tw[ax];
     pa.kf = (const T *)c->kf[ax];
     pa.nlines = (long)c->batch * (c->N / c->n[ax]);
@@ -1691,11 +1800,8 @@ static hipError_t enqueue_conc_adjoint(scft_ctx *c, double mu)
         const C *zt, *zu;
         if ((e = walk.next(tau, &zt, &zu)) != hipSuccess) return e;
         C *zb = slot_ptr<T>(c, zb0 + (tau & 1));
-        if (tau > 0) {
-            int br, bi;
-            step_blocks(c, tau, br, bi);
-            if ((e = enqueue_step<T>(c, slot_ptr<T>(c, zb0 + ((tau - 1) & 1)), zb, br, bi)) != hipSuccess) return e;
-        }
+        if (tau > 0 && (e = enqueue_step<T>(c, slot_ptr<T>(c, zb0 + ((tau - 1) & 1)), zb, tau, 0, 0)) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(k_adj_point<T>, gc, dim3(RT), 0, c->stream, zb, zt, zu, (const double *)lA,
                            (const double *)lB, tau, c->NsA, c->Ns, c->p.ds, c->N, c->batch, (const double *)c->Q,
                            (const double *)sc, gA, gB, tau == 0 ? 1 : 0);
@@ -1717,6 +1823,7 @@ extern "C" int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *
                                  double *gA, double *gB)
 {
     if (!ctx || !uA || !uB) return ctx ? set_err(ctx, SCFT_E_INVALID, "uA/uB are NULL") : SCFT_E_INVALID;
+    if (!ctx->blocks_uniform) return set_err(ctx, SCFT_E_UNSUPPORTED, "the adjoint needs one (NsA, NsB) for all instances");
     HIPX(ctx, hipSetDevice(ctx->device));
     int rc = ensure_adjoint_scratch(ctx);
     if (rc) return rc;
@@ -1808,6 +1915,7 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     if (!ctx) return SCFT_E_INVALID;
     if (!h || !w0A || !w0B || !target || n_iter < 1)
         return set_err(ctx, SCFT_E_INVALID, "scft_adjoint_loop: NULL input or n_iter < 1");
+    if (!ctx->blocks_uniform) return set_err(ctx, SCFT_E_UNSUPPORTED, "the adjoint needs one (NsA, NsB) for all instances");
     HIPX(ctx, hipSetDevice(ctx->device));
     int rc = ensure_adjoint_scratch(ctx);
     if (rc) return rc;
@@ -2020,5 +2128,247 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     }
     std::memcpy(grad_h, hbar.data(), 8 * BN);
     ctx->have_conc = 0;
+    return SCFT_OK;
+}
+
+// ===========================================================================
+// Heterogeneous parameter sweep (no reference counterpart: the reference's
+// batch shares chiN, f, box and grid, struct.h:42-68).  Instances are grouped
+// by grid size; each group is one context with per-instance chiN, fA (hence
+// NsA, NsB, Ns) and box length, all groups on one stream, and one SCFT
+// iteration of every group is captured as a single HIP graph.
+// ===========================================================================
+struct scft_sweep {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<scft_ctx *> groups;
+    std::vector<int> grp_of, idx_in;  // per input instance
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+static int sweep_err(scft_sweep *s, int code, const std::string &m)
+{
+    if (s) s->err = m;
+    else g_create_error = m;
+    return code;
+}
+
+static void sweep_free(scft_sweep *s)
+{
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+    for (scft_ctx *c : s->groups) scft_destroy(c);
+    s->groups.clear();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+extern "C" int scft_sweep_create(const scft_params *base, const scft_instance *inst, int count, int device,
+                                 scft_sweep **out)
+{
+    if (!out) return sweep_err(nullptr, SCFT_E_INVALID, "scft_sweep_create: out is NULL");
+    *out = nullptr;
+    if (!base || !inst || count < 1) return sweep_err(nullptr, SCFT_E_INVALID, "scft_sweep_create: empty sweep");
+    std::unique_ptr<scft_sweep> s(new scft_sweep());
+    s->device = device;
+    {
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+        if (e != hipSuccess) {
+            sweep_free(s.get());
+            return sweep_err(nullptr, SCFT_E_HIP, std::string("sweep stream/events: ") + hipGetErrorString(e));
+        }
+    }
+    std::vector<int> sizes;
+    for (int i = 0; i < count; ++i)
+        if (std::find(sizes.begin(), sizes.end(), inst[i].n) == sizes.end()) sizes.push_back(inst[i].n);
+    std::sort(sizes.begin(), sizes.end());
+    s->grp_of.assign(count, 0);
+    s->idx_in.assign(count, 0);
+    for (size_t g = 0; g < sizes.size(); ++g) {
+        std::vector<double> L, chi, f;
+        for (int i = 0; i < count; ++i)
+            if (inst[i].n == sizes[g]) {
+                s->grp_of[i] = (int)g;
+                s->idx_in[i] = (int)L.size();
+                L.push_back(inst[i].L);
+                chi.push_back(inst[i].chiN);
+                f.push_back(inst[i].fA);
+            }
+        scft_params p = *base;
+        p.nx = p.ny = p.nz = sizes[g];
+        p.lx = p.ly = p.lz = L[0];
+        p.chiN = chi[0];
+        p.fA = f[0];
+        p.batch = (int)L.size();
+        p.checkpoint = 0;
+        scft_ctx *c = nullptr;
+        int rc = create_ctx(&p, L.data(), chi.data(), f.data(), device, s->stream, &c);
+        if (rc) {
+            const std::string m = "size group " + std::to_string(sizes[g]) + ": " + g_create_error;
+            sweep_free(s.get());
+            return sweep_err(nullptr, rc, m);
+        }
+        s->groups.push_back(c);
+    }
+    *out = s.release();
+    return SCFT_OK;
+}
+
+extern "C" int scft_sweep_destroy(scft_sweep *s)
+{
+    if (!s) return SCFT_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    sweep_free(s);
+    delete s;
+    return SCFT_OK;
+}
+
+extern "C" const char *scft_sweep_last_error(const scft_sweep *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+extern "C" int scft_sweep_set_fields(scft_sweep *s, int i, const double *wA, const double *wB)
+{
+    if (!s || i < 0 || i >= (int)s->grp_of.size() || !wA || !wB)
+        return s ? sweep_err(s, SCFT_E_INVALID, "scft_sweep_set_fields: bad instance or NULL field") : SCFT_E_INVALID;
+    scft_ctx *c = s->groups[s->grp_of[i]];
+    for (long r = 0; r < c->N; ++r)
+        if (!std::isfinite(wA[r]) || !std::isfinite(wB[r]))
+            return sweep_err(s, SCFT_E_NONFINITE, "non-finite field value in instance " + std::to_string(i));
+    const size_t off = (size_t)s->idx_in[i] * c->N;
+    hipError_t e = hipSetDevice(s->device);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->wA + off, wA, 8 * c->N, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->wB + off, wB, 8 * c->N, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess && c->and_cnt) {
+        const long zero = 0;  // new fields restart the group's Anderson history
+        e = hipMemcpyAsync(c->and_cnt, &zero, sizeof(long), hipMemcpyHostToDevice, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("scft_sweep_set_fields: ") + hipGetErrorString(e));
+    return SCFT_OK;
+}
+
+extern "C" int scft_sweep_get_fields(scft_sweep *s, int i, double *phiA, double *phiB, double *wA, double *wB)
+{
+    if (!s || i < 0 || i >= (int)s->grp_of.size())
+        return s ? sweep_err(s, SCFT_E_INVALID, "scft_sweep_get_fields: bad instance") : SCFT_E_INVALID;
+    scft_ctx *c = s->groups[s->grp_of[i]];
+    const size_t off = (size_t)s->idx_in[i] * c->N;
+    hipError_t e = hipSetDevice(s->device);
+    if (e == hipSuccess && phiA) e = hipMemcpyAsync(phiA, c->phiA + off, 8 * c->N, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && phiB) e = hipMemcpyAsync(phiB, c->phiB + off, 8 * c->N, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && wA) e = hipMemcpyAsync(wA, c->wA + off, 8 * c->N, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && wB) e = hipMemcpyAsync(wB, c->wB + off, 8 * c->N, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("scft_sweep_get_fields: ") + hipGetErrorString(e));
+    return SCFT_OK;
+}
+
+static hipError_t sweep_launch(scft_sweep *s)
+{
+    if (!s->exec) {
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t e1 = hipSuccess;
+        for (scft_ctx *c : s->groups)
+            if (e1 == hipSuccess) e1 = enqueue_iteration(c);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s->stream, &g);
+        if (e1 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return e1;
+        }
+        if (e != hipSuccess) return e;
+        s->graph = g;
+        e = hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            s->exec = nullptr;
+            return e;
+        }
+    }
+    return hipGraphLaunch(s->exec, s->stream);
+}
+
+static int sweep_collect_fe(scft_sweep *s, double *fe)
+{
+    const int count = (int)s->grp_of.size();
+    std::vector<std::vector<double>> gfe(s->groups.size());
+    for (size_t g = 0; g < s->groups.size(); ++g) {
+        scft_ctx *c = s->groups[g];
+        gfe[g].resize(5 * c->batch);
+        hipError_t e = hipMemcpyAsync(gfe[g].data(), c->fe, 8 * gfe[g].size(), hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("free energy copy: ") + hipGetErrorString(e));
+        c->have_fe = 1;
+        c->have_conc = 0;
+    }
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("sweep: ") + hipGetErrorString(e));
+    int rc = SCFT_OK;
+    for (int i = 0; i < count; ++i) {
+        const double *f = gfe[s->grp_of[i]].data() + 5 * s->idx_in[i];
+        if (fe) std::memcpy(fe + 5 * i, f, 5 * sizeof(double));
+        if (!std::isfinite(f[0]) && rc == SCFT_OK)
+            rc = sweep_err(s, SCFT_E_NONFINITE, "non-finite free energy in instance " + std::to_string(i));
+    }
+    return rc;
+}
+
+extern "C" int scft_sweep_iterate(scft_sweep *s, int n_iter, double *fe)
+{
+    if (!s || n_iter < 0) return SCFT_E_INVALID;
+    hipError_t e = hipSetDevice(s->device);
+    for (int it = 0; it < n_iter && e == hipSuccess; ++it) e = sweep_launch(s);
+    if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("sweep iteration: ") + hipGetErrorString(e));
+    if (n_iter == 0) return SCFT_OK;
+    return sweep_collect_fe(s, fe);
+}
+
+extern "C" int scft_sweep_time_iterations(scft_sweep *s, int n_iter, float *ms)
+{
+    if (!s || !ms || n_iter < 1) return SCFT_E_INVALID;
+    hipError_t e = hipSetDevice(s->device);
+    if (e == hipSuccess && !s->exec) {  // capture outside the timed region
+        e = sweep_launch(s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(s->ev0, s->stream);
+    for (int it = 0; it < n_iter && e == hipSuccess; ++it) e = sweep_launch(s);
+    if (e == hipSuccess) e = hipEventRecord(s->ev1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, s->ev0, s->ev1);
+    if (e != hipSuccess) return sweep_err(s, SCFT_E_HIP, std::string("sweep timing: ") + hipGetErrorString(e));
+    return sweep_collect_fe(s, nullptr);
+}
+
+extern "C" int scft_sweep_autotune(scft_sweep *s, int reps)
+{
+    if (!s || reps < 1) return SCFT_E_INVALID;
+    for (scft_ctx *c : s->groups) {
+        int rc = scft_autotune(c, reps, nullptr, nullptr);
+        if (rc) return sweep_err(s, rc, std::string("autotune: ") + c->err);
+    }
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+    return SCFT_OK;
+}
+
+extern "C" int scft_sweep_size(const scft_sweep *s, int i, long *N, int *NsA, int *NsB)
+{
+    if (!s || i < 0 || i >= (int)s->grp_of.size()) return SCFT_E_INVALID;
+    const scft_ctx *c = s->groups[s->grp_of[i]];
+    const int2 ns = c->nsab_h[s->idx_in[i]];
+    if (N) *N = c->N;
+    if (NsA) *NsA = ns.x;
+    if (NsB) *NsB = ns.y;
     return SCFT_OK;
 }

@@ -83,3 +83,20 @@ def workload_fields(name, n=None):
     w = WORKLOADS[name]
     n = n or w["n"]
     return seeded_fields(n, n, n, w["L"], w["L"], w["L"], w["kind"], w["amp"], w["noise"], w["seed"])
+
+
+def sweep_instances(count=256, seed=4, sizes=(32, 48, 64), chi=(12.0, 30.0), f=(0.25, 0.5), L=(3.5, 5.0)):
+    """Seeded instance list (n, L, chiN, fA) of the phase-diagram sweep:
+    grid drawn from `sizes`, chiN, fA and L uniform in their ranges."""
+    u = splitmix_uniform(seed, 4 * count).reshape(count, 4)
+    out = []
+    for i in range(count):
+        n = sizes[min(int(u[i, 0] * len(sizes)), len(sizes) - 1)]
+        out.append((n, L[0] + (L[1] - L[0]) * u[i, 3], chi[0] + (chi[1] - chi[0]) * u[i, 1],
+                    f[0] + (f[1] - f[0]) * u[i, 2]))
+    return out
+
+
+def sweep_fields(instances, noise=0.1, seed=400):
+    """seeded +-noise fields of every sweep instance (instance i uses stream seed + i)"""
+    return [seeded_fields(n, n, n, L, L, L, "none", 0.0, noise, seed + i) for i, (n, L, c, f) in enumerate(instances)]
