@@ -51,12 +51,17 @@ def iter_bytes(N, Ns, b):
     return N * (b * (6 * Ns + 2 * (Ns + 2)) + 64)
 
 
-def run_form(pkg, name, prec, steps, warmup):
+def run_form(pkg, name, prec, steps, warmup, choice=None):
     p = workload_params(pkg, name, prec)
     wA, wB = pkg.fields.workload_fields(name)
     s = pkg.Solver(p)
-    choice, tms = s.autotune(20)
-    log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} autotune choice={choice} pass_ms={tms}")
+    if choice is None:
+        choice, tms = s.autotune(20)
+        log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} autotune choice={choice} pass_ms={tms}")
+    else:
+        s.set_tiling(choice)
+        tms = None
+        log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} tiling set to {choice}")
     s.tuning = (choice, tms)
     s.set_fields(wA, wB)
     log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} warmup {warmup}")
@@ -85,6 +90,26 @@ def pass_roofline(s, prec, reps=50):
     bytes_per_launch = 4 * b * s.N  # read + write of the complex field (2b bytes per point each way)
     achieved = bytes_per_launch / (out[dom] * 1e-3) / 1e9
     return dom, out, bytes_per_launch, achieved
+
+
+def calibration_copy(nbytes, reps):
+    """Device-to-device copies of a known size (the runtime's blit kernel), so
+    PMC byte counters can be scaled to bytes: factor = nbytes / counter."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src = ctypes.c_void_p()
+    dst = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 0, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"[bench] calibration copy: {reps} x {nbytes >> 20} MiB device to device")
 
 
 def cpu_baseline(pkg, name, budget_s=12.0, max_iters=4):
@@ -125,6 +150,11 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c3_fp32", "c3_fp64"],
                     help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
+    ap.add_argument("--choice", default=None,
+                    help="with --profile: tiling 'x,y,z,plane' to apply instead of autotuning "
+                         "(a profiler distorts the autotuner's timings)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="with --profile: also copy 64 MiB -> 64 MiB three times (PMC byte calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this solver runs one GPU per process and this build measures one GPU only; "
@@ -135,9 +165,15 @@ def main():
     if a.profile:
         wl, form = a.profile.split("_")
         nm = {"c2": "c2_cylinder_64", "c3": "c3_gyroid_128"}[wl]
-        s, ms, fe, _ = run_form(pkg, nm, pkg.FP32 if form == "fp32" else pkg.FP64, a.steps, a.warmup)
-        print(json.dumps({"profile": a.profile, "ms_per_iter": ms, "iters_per_s": 1e3 / ms, "F": fe[0]}))
+        prec = pkg.FP32 if form == "fp32" else pkg.FP64
+        choice = [int(c) for c in a.choice.split(",")] if a.choice else None
+        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice)
+        dom, passes, bpl, ach = pass_roofline(s, prec)
+        print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "ms_per_iter": ms, "iters_per_s": 1e3 / ms,
+                          "F": fe[0], "pass_ms": passes, "dominant": dom}))
         s.close()
+        if a.calib_copy:
+            calibration_copy(64 << 20, 3)
         return
     name = "c2_cylinder_64"
     s32, ms32, fe32, phi32 = run_form(pkg, name, pkg.FP32, a.steps, a.warmup)
@@ -203,6 +239,13 @@ def main():
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, host cores)")
         cpu = cpu_baseline(pkg, name)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        key = f"c2_fp32:{dom}"
+        traffic = tr.get(key)
     out = {
         "metric": "SCFT iters/s at 64^3,128^3 Ns=200 (FP32 vs FP64 form), %HBM peak, dF vs FP64",
         "value": it32,
@@ -219,9 +262,10 @@ def main():
         "config": {"workload": "c2 cylinder: AB diblock f=0.3 chiN=20, 64^3, L=4.2 Rg, ds=0.005 "
                                "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
                    "batch": 1},
-        "roofline": {"bound": "hbm", "kernel": f"k_line_conv {dom}-pass (fp32)", "achieved": ach,
+        "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0]},
+        "roofline": {"bound": "hbm", "kernel": ("k_plane_xy (fp32)" if dom == "xy_plane" else f"k_line_conv {dom}-pass (fp32)"), "achieved": ach,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
-                     "traffic": None, "bytes_per_launch": bpl, "ms_per_launch": passes32[dom]},
+                     "traffic": traffic, "bytes_per_launch": bpl, "ms_per_launch": passes32[dom]},
         "cpu_baseline": cpu,
         "forms": forms,
     }

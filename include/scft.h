@@ -168,6 +168,13 @@ int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const d
    used (0/1); ms (may be NULL): float[4] = best x, y, z pass and plane times. */
 int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 
+/* Apply a tiling choice of the form scft_autotune returns (int[4]: variant of
+   the x, y, z pass, xy-plane kernel 0/1) without timing anything, e.g. one
+   cached from an earlier scft_autotune on the same grid.  SCFT_E_UNSUPPORTED if
+   a variant or the plane kernel does not fit this grid.  No reference
+   counterpart (its launch shapes are fixed, init_cuda.cu:21-80). */
+int scft_set_tiling(scft_ctx *ctx, const int *choice);
+
 /* ---- heterogeneous parameter sweep (no reference counterpart: the
    reference's batch shares chiN, f, box and grid, struct.h:42-68) ----
    Independent instances with their own cubic grid n^3, box L^3, chiN and fA

@@ -53,7 +53,7 @@ ABI_SYMBOLS = (
     "scft_default_params", "scft_contour_counts", "scft_create", "scft_destroy",
     "scft_last_error", "scft_set_fields", "scft_get_fields", "scft_propagate",
     "scft_concentrations", "scft_update_fields", "scft_free_energy", "scft_iterate",
-    "scft_time_iterations", "scft_time_pass", "scft_autotune", "scft_sizes",
+    "scft_time_iterations", "scft_time_pass", "scft_autotune", "scft_set_tiling", "scft_sizes",
     "scft_conc_adjoint", "scft_adjoint_loop",
     "scft_sweep_create", "scft_sweep_destroy", "scft_sweep_last_error", "scft_sweep_set_fields",
     "scft_sweep_get_fields", "scft_sweep_iterate", "scft_sweep_time_iterations", "scft_sweep_autotune",
@@ -143,6 +143,7 @@ def lib():
                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             "scft_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_float)]),
+            "scft_set_tiling": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
         }
@@ -313,6 +314,11 @@ class Solver:
         ms = (ctypes.c_float * 4)()
         self._check(lib().scft_autotune(self._h, int(reps), ch, ms))
         return list(ch), list(ms)
+
+    def set_tiling(self, choice):
+        """Apply a tiling choice [x, y, z variant, plane 0/1] (as autotune returns)."""
+        ch = (ctypes.c_int * 4)(*[int(c) for c in choice])
+        self._check(lib().scft_set_tiling(self._h, ch))
 
     def time_pass(self, axis, reps):
         ms = ctypes.c_float()

@@ -1756,6 +1756,34 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     return SCFT_OK;
 }
 
+extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
+{
+    if (!ctx || !choice) return SCFT_E_INVALID;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    for (int a = 0; a < 3; ++a)
+        if (choice[a] < 0 || choice[a] >= PASS_VARIANTS)
+            return set_err(ctx, SCFT_E_INVALID, "tiling variant must be 0.." + std::to_string(PASS_VARIANTS - 1));
+    const bool fits = ctx->n[0] == ctx->n[1] && ctx->n[0] > 1 &&
+                      (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]));
+    if (choice[3] != 0 && !fits) return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
+    for (int a = 0; a < 3; ++a) {  // one trial launch: a variant whose LDS does not fit fails here
+        if (!ctx->active[a] || choice[a] == 0) continue;
+        float t = 0.0f;
+        if (time_pass_variant(ctx, a, 1, choice[a], &t) != SCFT_OK) {
+            (void)hipGetLastError();
+            return set_err(ctx, SCFT_E_UNSUPPORTED, "tiling variant " + std::to_string(choice[a]) +
+                                                        " does not fit axis " + std::to_string(a));
+        }
+    }
+    for (int a = 0; a < 3; ++a) ctx->pass_variant[a] = choice[a];
+    ctx->use_plane = choice[3] != 0;
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    return SCFT_OK;
+}
+
 // ===========================================================================
 // Discrete adjoint (no reference counterpart)
 // ===========================================================================

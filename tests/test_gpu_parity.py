@@ -292,3 +292,24 @@ def test_autotune_keeps_results(pkg):
             out.append(fe)
     # plane fusion and tiling change the order of operations only within FP32 rounding
     assert np.max(np.abs(out[0] - out[1])) < 2e-6
+
+
+@pytest.mark.gpu
+def test_set_tiling_matches_autotune_choice(pkg):
+    """scft_set_tiling applied to a fresh context reproduces the results of the
+    context that ran scft_autotune with the same choice bit for bit; bad
+    variants are rejected before any launch."""
+    p = _params(pkg, 32, 4.0, 15.0, 0.4, 0.02, prec=0)
+    wA, wB = pkg.fields.seeded_fields(32, 32, 32, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.2, 63)
+    with pkg.Solver(p) as s:
+        choice, _ = s.autotune(3)
+        s.set_fields(wA, wB)
+        _, fe0 = s.iterate(3)
+    with pkg.Solver(p) as s:
+        s.set_tiling(choice)
+        s.set_fields(wA, wB)
+        _, fe1 = s.iterate(3)
+        with pytest.raises(pkg.ScftError):
+            s.set_tiling([0, 0, 9, 0])
+    print("choice", choice, flush=True)
+    assert np.array_equal(fe0, fe1)
