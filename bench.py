@@ -73,6 +73,26 @@ def run_form(pkg, name, prec, steps, warmup, choice=None):
     return s, ms / steps, fe[0].copy(), phiA[0].copy()
 
 
+def single_eval_error(pkg, name, wA, wB):
+    """One evaluation (getConc, field update, F) of the FP32 and FP64 forms on
+    identical fields: |dQ|/Q, |dF| and max|dphi| of the FP32 form against the
+    FP64 form (which matches the FP64 oracle to ~1e-12 on every tested grid)."""
+    out = {}
+    for prec in (pkg.FP64, pkg.FP32):
+        p = workload_params(pkg, name, prec)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations()[0]
+            phiA, phiB, _, _ = s.get_fields()
+            s.update_fields()
+            fe = s.free_energy()[0]
+            out[prec] = (Q, fe[0], phiA[0].copy(), phiB[0].copy())
+    q64, f64, a64, b64 = out[pkg.FP64]
+    q32, f32, a32, b32 = out[pkg.FP32]
+    return {"relQ": abs(q32 - q64) / q64, "dF": abs(f32 - f64), "F_fp64": f64,
+            "max_dphiA": float(np.max(np.abs(a32 - a64))), "max_dphiB": float(np.max(np.abs(b32 - b64)))}
+
+
 def pass_roofline(s, prec, reps=50):
     """Mean duration of each axis pass (HIP events around `reps` back-to-back
     launches on the solver stream) and the dominant one's bandwidth."""
@@ -80,12 +100,13 @@ def pass_roofline(s, prec, reps=50):
     out = {}
     for axis, nm in enumerate("xyz"):
         out[nm] = s.time_pass(axis, reps)
-    try:
-        out["xy_plane"] = s.time_pass(3, reps)
-    except Exception:
-        pass
-    plane = bool(getattr(s, "tuning", ([0, 0, 0, 0], None))[0][3])
-    used = {k: v for k, v in out.items() if k in (("xy_plane", "z") if plane else ("x", "y", "z"))}
+    for form, nm in ((3, "xy_plane"), (4, "xy_split")):
+        try:
+            out[nm] = s.time_pass(form, reps)
+        except Exception:
+            pass
+    plane = getattr(s, "tuning", ([0, 0, 0, 0], None))[0][3]
+    used = {k: v for k, v in out.items() if k in {0: ("x", "y", "z"), 1: ("xy_plane", "z"), 2: ("xy_split", "z")}[plane]}
     dom = max(used, key=used.get)
     bytes_per_launch = 4 * b * s.N  # read + write of the complex field (2b bytes per point each way)
     achieved = bytes_per_launch / (out[dom] * 1e-3) / 1e9
@@ -189,32 +210,37 @@ def main():
             "fp32_iters_per_s": it32, "fp64_iters_per_s": it64, "ratio_fp32_over_fp64": it32 / it64,
             "fp32_iter_hbm_frac": iter_bytes(N, Ns, 4) * it32 / 1e9 / HBM_PEAK_GBPS,
             "fp64_iter_hbm_frac": iter_bytes(N, Ns, 8) * it64 / 1e9 / HBM_PEAK_GBPS,
-            "dF_fp32_vs_fp64": abs(fe32[0] - fe64[0]),
-            "max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
+            "trajectory_dF_fp32_vs_fp64": abs(fe32[0] - fe64[0]),
+            "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
             "F_fp64": fe64[0], "iterations_compared": a.warmup + a.steps,
             "pass_ms_fp32": passes32, "pass_ms_fp64": passes64,
         }
     }
+    _, _, wa64, wb64 = s64.get_fields()
     s32.close()
     s64.close()
+    forms["c2_64"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name, wa64[0], wb64[0])
     if not a.no_128:
         name3 = "c3_gyroid_128"
         t32, m32, f32, p32 = run_form(pkg, name3, pkg.FP32, max(5, a.steps // 2), 3)
         t64, m64, f64, p64 = run_form(pkg, name3, pkg.FP64, max(5, a.steps // 2), 3)
         _, pass3_32, _, _ = pass_roofline(t32, 1, reps=20)
         _, pass3_64, _, _ = pass_roofline(t64, 0, reps=20)
+        _, _, wa64, wb64 = t64.get_fields()
         N3 = t32.N
         Ns3 = t32.NsA + t32.NsB
         forms["c3_128"] = {
             "fp32_iters_per_s": 1e3 / m32, "fp64_iters_per_s": 1e3 / m64, "ratio_fp32_over_fp64": m64 / m32,
             "fp32_iter_hbm_frac": iter_bytes(N3, Ns3, 4) * (1e3 / m32) / 1e9 / HBM_PEAK_GBPS,
             "fp64_iter_hbm_frac": iter_bytes(N3, Ns3, 8) * (1e3 / m64) / 1e9 / HBM_PEAK_GBPS,
-            "dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
-            "max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
+            "trajectory_dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
+            "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
             "pass_ms_fp32": pass3_32, "pass_ms_fp64": pass3_64,
         }
         t32.close()
         t64.close()
+        log("[bench] c3 one evaluation of both forms on the FP64 run's final fields")
+        forms["c3_128"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0])
     if not a.no_sweep:
         log("[bench] c4 sweep: 256 instances, grids 32/48/64, FP64")
         inst = pkg.fields.sweep_instances(count=256, seed=4)
@@ -263,7 +289,8 @@ def main():
                                "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
                    "batch": 1},
         "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0]},
-        "roofline": {"bound": "hbm", "kernel": ("k_plane_xy (fp32)" if dom == "xy_plane" else f"k_line_conv {dom}-pass (fp32)"), "achieved": ach,
+        "roofline": {"bound": "hbm", "kernel": {"xy_plane": "k_plane_xy (fp32)", "xy_split": "k_plane_split (fp32)"}.get(
+                         dom, f"k_line_conv {dom}-pass (fp32)"), "achieved": ach,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
                      "traffic": traffic, "bytes_per_launch": bpl, "ms_per_launch": passes32[dom]},
         "cpu_baseline": cpu,

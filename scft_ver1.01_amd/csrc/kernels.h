@@ -18,38 +18,40 @@ namespace scft {
 
 enum Axis { AX_X = 0, AX_Y = 1, AX_Z = 2 };
 
+// Entry type of a k-factor table (the propagator precision).
+template <typename T> struct kfac { using type = T; };
+template <typename C, typename T> __device__ __forceinline__ C kf_apply(C v, T k) { return C{v.x * k, v.y * k}; }
+
 // One batched 1-D convolution pass along an axis:
 //   line <- IDFT[ kf(k) * DFT[ line ] ]      (unnormalised DFTs, kf(0) = 1)
-// pre:  the loaded value is multiplied by (wre, wim) part-wise first,
-// post: the result is multiplied by (wre_post, wim_post) part-wise before the
+// pre:  the loaded value is multiplied part-wise by the pre W factors first,
+// post: the result is multiplied part-wise by the post W factors before the
 //       store (these carry the 1/N of the inverse transforms).
-// Per-instance mode (step > 0): the W factors of each line's instance are
+// W = [pre A, pre B, post A, post B] ([4][BN]); step == 0 takes the four
+// factors at the element offsets wo[] (pre real, pre imag, post real, post
+// imag) for every line.  Per-instance mode (step > 0): each line's blocks are
 // chosen in the kernel from nsab[instance] = (NsA, NsB) at contour step
 // `step` (real part: block A while step <= NsA; imaginary part: block B while
-// step <= NsB) out of W = [pre A, pre B, post A, post B] ([4][batch][N]), and
-// lines of instances whose chain is shorter than `step` are left untouched.
-// step == 0 uses wre..wim_post as given for every line.  kf of instance b is
-// kf + b * kf_stride.
+// step <= NsB), and lines of instances whose chain is shorter than `step` are
+// left untouched.  kf of instance b is kf + b * kf_stride.
+// Every element offset is 32-bit: the host keeps 4*BN*sizeof(T) < 2^32.
 template <typename T>
 struct PassArgs {
     using C = typename cplx<T>::type;
     const C *src;
     C *dst;
-    const T *wre;
-    const T *wim;
-    const T *wre_post;
-    const T *wim_post;
-    const T *W;          // per-instance mode: the four W arrays
+    const T *W;          // the four W arrays
+    unsigned wo[4];      // step == 0: element offsets of the four factors in W
     const int2 *nsab;    // per-instance mode: (NsA, NsB) per instance
-    const C *tw;   // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
-    const T *kf;   // n entries per instance (kf_stride apart, 0 = shared)
-    long kf_stride;
-    long nlines;   // lines in this launch (batch * N / n)
-    long N;        // points per instance
-    long BN;       // batch * N
+    const C *tw;         // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
+    const typename kfac<T>::type *kf;  // n entries per instance (kf_stride apart, 0 = shared)
+    unsigned kf_stride;
+    unsigned nlines;     // lines in this launch (batch * N / n)
+    unsigned N;          // points per instance
+    unsigned BN;         // batch * N
     int nx, ny, nz;
     int pre, post;
-    int step;      // contour step (per-instance mode) or 0
+    int step;            // contour step (per-instance mode) or 0
 };
 
 // true if the pass kernels have an instantiation for line length n
@@ -65,8 +67,17 @@ bool plane_supported(int n);
 // x and y passes of every plane (nplanes = batch * nz) in one launch; `a`
 // carries the x tables, twy/kfy the y tables
 template <typename T>
-hipError_t launch_plane(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy, const T *kfy, long nplanes,
+hipError_t launch_plane(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
+                        const typename kfac<T>::type *kfy, long nplanes,
                         hipStream_t s);
+// true if the split xy-plane kernel (one workgroup per plane and real part)
+// exists for nx = ny = n in precision T
+template <typename T>
+bool split_supported(int n);
+// the split form of launch_plane: 2 * 8 * ceil(nplanes / 8) workgroups
+template <typename T>
+hipError_t launch_split(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
+                        const typename kfac<T>::type *kfy, long nplanes, hipStream_t s);
 // the (A, B, lines-per-workgroup, threads) tiling used for (axis, n)
 void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *threads);
 

@@ -6,4 +6,7 @@ template hipError_t launch_pass<float>(int, int, int, const PassArgs<float> &, h
 template bool plane_supported<float>(int);
 template hipError_t launch_plane<float>(int, const PassArgs<float> &, const cplx<float>::type *, const float *, long,
                                         hipStream_t);
+template bool split_supported<float>(int);
+template hipError_t launch_split<float>(int, const PassArgs<float> &, const cplx<float>::type *, const float *, long,
+                                        hipStream_t);
 }  // namespace scft

@@ -37,4 +37,7 @@ void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *
     }
 }
 
+template bool split_supported<double>(int);
+template hipError_t launch_split<double>(int, const PassArgs<double> &, const cplx<double>::type *, const double *, long,
+                                        hipStream_t);
 }  // namespace scft

@@ -720,7 +720,7 @@ struct scft_ctx {
     int n[3] = {1, 1, 1};
     int active[3] = {0, 0, 0};
     int first_axis = -1, last_axis = -1;
-    int use_plane = 0;  // x and y passes fused in one xy-plane kernel
+    int use_plane = 0;  // x and y passes in one kernel: 0 no, 1 xy-plane, 2 split xy-plane
     int ckpt = 0;       // sqrt(Ns) checkpointing of the slice store
     int ck_K = 1, ck_n = 1;  // checkpoint spacing and count
     // adjoint scratch (allocated on first use): uA, uB, lamA, lamB, gA, gB, phiA, phiB
@@ -932,6 +932,16 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
     return SCFT_OK;
 }
 
+// true if xy-plane form `form` (1 xy-plane, 2 split xy-plane) has a kernel for
+// an nx x ny plane in this precision that fits a CU's LDS
+static bool plane_fits(bool fp32, int nx, int ny, int form)
+{
+    if (nx != ny || nx < 2) return false;
+    if (form == 1) return fp32 ? plane_supported<float>(nx) : plane_supported<double>(nx);
+    if (form == 2) return fp32 ? split_supported<float>(nx) : split_supported<double>(nx);
+    return false;
+}
+
 // Context constructor.  inst_L / inst_chiN / inst_fA (length p.batch, each may
 // be null) give per-instance box lengths (cubic L, L, L), chiN and fA in place
 // of the uniform ones in p; stream, when not null, is shared (not owned).
@@ -1021,6 +1031,11 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     c->fp32 = p.precision == SCFT_FP32;
     c->csize = c->fp32 ? 8 : 16;
     c->tsize = c->fp32 ? 4 : 8;
+    // the pass kernels address the four W arrays ([4][batch][N]) with 32-bit byte offsets
+    if ((double)4 * c->tsize * c->batch * c->N >= 4294967296.0)
+        return set_err(nullptr, SCFT_E_UNSUPPORTED,
+                       "batch * N too large for one context (4 * batch * N * sizeof(real) must stay below 2^32 "
+                       "bytes): split the batch over several contexts");
     for (int a = 0; a < 3; ++a) {
         c->n[a] = n[a];
         c->active[a] = n[a] > 1;
@@ -1031,12 +1046,11 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     }
     // reduction blocks per instance: ~2 points per thread, at most 2048 partials
     {
-        const bool fits = p.nx == p.ny && p.nx > 1 &&
-                          (c->fp32 ? plane_supported<float>(p.nx) : plane_supported<double>(p.nx));
-        if (p.xy_plane == 1 && !fits)
-            return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane = 1 needs nx == ny with a plane kernel that "
-                                                        "fits LDS (16, 32, 48, 64; 128 in FP32)");
-        c->use_plane = (p.xy_plane == 1) && fits;  // auto: decided by scft_autotune
+        if (p.xy_plane < -1 || p.xy_plane > 2) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..2");
+        if (p.xy_plane > 0 && !plane_fits(c->fp32, p.nx, p.ny, p.xy_plane))
+            return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane needs nx == ny with a plane kernel that fits LDS "
+                                                        "(16, 32, 48, 64; 128 except the FP64 xy-plane kernel)");
+        c->use_plane = p.xy_plane > 0 ? p.xy_plane : 0;  // auto: decided by scft_autotune
     }
     if (p.tiling < -1 || p.tiling >= PASS_VARIANTS)
         return set_err(nullptr, SCFT_E_INVALID, "tiling must be -1 or 0.." + std::to_string(PASS_VARIANTS - 1));
@@ -1226,25 +1240,23 @@ static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, t
     const size_t BN = (size_t)c->batch * c->N;
     C *work = (C *)c->work;
     const T *W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
-    const T *wre = W + br * BN, *wim = W + bi * BN;
-    const T *post_r = W + (2 + br) * BN, *post_i = W + (2 + bi) * BN;
     auto args = [&](int a, bool first, bool last) {
         PassArgs<T> pa;
         pa.src = first ? src : work;
         pa.dst = last ? dst : work;
-        pa.wre = wre;
-        pa.wim = wim;
-        pa.wre_post = post_r;
-        pa.wim_post = post_i;
         pa.W = W;
+        pa.wo[0] = (unsigned)(br * BN);
+        pa.wo[1] = (unsigned)(bi * BN);
+        pa.wo[2] = (unsigned)((2 + br) * BN);
+        pa.wo[3] = (unsigned)((2 + bi) * BN);
         pa.nsab = (const int2 *)c->nsab;
         pa.step = t;
-        pa.BN = (long)BN;
+        pa.BN = (unsigned)BN;
         pa.tw = (const C *)c->tw[a];
-        pa.kf = (const T *)c->kf[a];
-        pa.kf_stride = c->kf_per_inst ? c->n[a] : 0;
-        pa.nlines = (long)c->batch * (c->N / c->n[a]);
-        pa.N = c->N;
+        pa.kf = (const typename kfac<T>::type *)c->kf[a];
+        pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[a] : 0u;
+        pa.nlines = (unsigned)(c->batch * (c->N / c->n[a]));
+        pa.N = (unsigned)c->N;
         pa.nx = c->n[0];
         pa.ny = c->n[1];
         pa.nz = c->n[2];
@@ -1254,8 +1266,11 @@ static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, t
     };
     if (c->use_plane) {
         const bool zact = c->active[2];
-        hipError_t e = launch_plane<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], (const T *)c->kf[1],
-                                       (long)c->batch * c->n[2], c->stream);
+        const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
+        const long np = (long)c->batch * c->n[2];
+        hipError_t e = c->use_plane == 2
+                           ? launch_split<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], kfy, np, c->stream)
+                           : launch_plane<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], kfy, np, c->stream);
         if (e != hipSuccess || !zact) return e;
         return launch_pass<T>(2, c->n[2], c->pass_variant[2], args(2, false, true), c->stream);
     }
@@ -1630,29 +1645,28 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     PassArgs<T> pa;
     pa.src = (const C *)c->work;
     pa.dst = (C *)c->work;
-    pa.wre = (const T *)c->wdz;
-    pa.wim = (const T *)c->wdz;
-    pa.wre_post = (const T *)c->wdz;
-    pa.wim_post = (const T *)c->wdz;
     pa.W = (const T *)c->wdz;
+    pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
     pa.nsab = (const int2 *)c->nsab;
     pa.step = 0;
-    pa.BN = (long)c->batch * c->N;
-    const int ax = axis == 3 ? 0 : axis;
-    pa.kf_stride = c->kf_per_inst ? c->n[ax] : 0;
+    pa.BN = (unsigned)(c->batch * c->N);
+    const int ax = axis >= 3 ? 0 : axis;
+    pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[ax] : 0u;
     pa.tw = (const C *)c->tw[ax];
-    pa.kf = (const T *)c->kf[ax];
-    pa.nlines = (long)c->batch * (c->N / c->n[ax]);
-    pa.N = c->N;
+    pa.kf = (const typename kfac<T>::type *)c->kf[ax];
+    pa.nlines = (unsigned)(c->batch * (c->N / c->n[ax]));
+    pa.N = (unsigned)c->N;
     pa.nx = c->n[0];
     pa.ny = c->n[1];
     pa.nz = c->n[2];
     pa.pre = 0;
     pa.post = 0;
+    const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
+    const long np = (long)c->batch * c->n[2];
     for (int r = 0; r < reps; ++r) {
-        hipError_t e = axis == 3 ? launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], (const T *)c->kf[1],
-                                                   (long)c->batch * c->n[2], c->stream)
-                                 : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
+        hipError_t e = axis == 4   ? launch_split<T>(c->n[0], pa, (const C *)c->tw[1], kfy, np, c->stream)
+                       : axis == 3 ? launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], kfy, np, c->stream)
+                                   : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1686,10 +1700,9 @@ static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, flo
 
 extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
-    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 3) return SCFT_E_INVALID;
-    if (axis == 3 && !(ctx->n[0] == ctx->n[1] &&
-                       (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]))))
-        return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel for this grid");
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 4) return SCFT_E_INVALID;
+    if (axis >= 3 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], axis - 2))
+        return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel of this form for this grid");
     if (axis < 3 && !ctx->active[axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
     HIPX(ctx, hipSetDevice(ctx->device));
     return time_pass_variant(ctx, axis, reps, axis < 3 ? ctx->pass_variant[axis] : 0, ms);
@@ -1726,15 +1739,23 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
         per_axis[a] = bt;
         best_sep += bt;
     }
-    int plane = 0;
-    const bool fits = ctx->n[0] == ctx->n[1] && ctx->n[0] > 1 &&
-                      (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]));
+    // xy-plane forms: forced by p.xy_plane > 0, off at 0, timed against the
+    // separate x and y passes at -1
+    int plane = ctx->p.xy_plane > 0 ? ctx->p.xy_plane : 0;
     float tplane = 0.0f;
-    if (fits && ctx->p.xy_plane != 0) {
-        int rc = time_pass_variant(ctx, 3, reps, 0, &tplane);
+    const float tz = ctx->active[2] ? per_axis[2] : 0.0f;
+    float best = best_sep;
+    for (int form = 1; form <= 2 && ctx->p.xy_plane != 0; ++form) {
+        if (!plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form)) continue;
+        if (ctx->p.xy_plane > 0 && form != ctx->p.xy_plane) continue;
+        float t = 0.0f;
+        int rc = time_pass_variant(ctx, 2 + form, reps, 0, &t);
         if (rc) return rc;
-        const float with_plane = tplane + (ctx->active[2] ? per_axis[2] : 0.0f);
-        plane = (ctx->p.xy_plane == 1) || with_plane < best_sep;
+        if (ctx->p.xy_plane > 0 || t + tz < best) {
+            best = t + tz;
+            plane = form;
+            tplane = t;
+        }
     }
     ctx->use_plane = plane;
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -1763,9 +1784,9 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
     for (int a = 0; a < 3; ++a)
         if (choice[a] < 0 || choice[a] >= PASS_VARIANTS)
             return set_err(ctx, SCFT_E_INVALID, "tiling variant must be 0.." + std::to_string(PASS_VARIANTS - 1));
-    const bool fits = ctx->n[0] == ctx->n[1] && ctx->n[0] > 1 &&
-                      (ctx->fp32 ? plane_supported<float>(ctx->n[0]) : plane_supported<double>(ctx->n[0]));
-    if (choice[3] != 0 && !fits) return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
+    if (choice[3] < 0 || choice[3] > 2) return set_err(ctx, SCFT_E_INVALID, "plane form must be 0, 1 or 2");
+    if (choice[3] != 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], choice[3]))
+        return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
     for (int a = 0; a < 3; ++a) {  // one trial launch: a variant whose LDS does not fit fails here
         if (!ctx->active[a] || choice[a] == 0) continue;
         float t = 0.0f;
@@ -1776,7 +1797,7 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
         }
     }
     for (int a = 0; a < 3; ++a) ctx->pass_variant[a] = choice[a];
-    ctx->use_plane = choice[3] != 0;
+    ctx->use_plane = choice[3];
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr;

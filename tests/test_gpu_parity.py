@@ -236,12 +236,15 @@ def test_q_underflow_reported(pkg):
 
 
 @pytest.mark.parametrize("shape,prec", [((32, 32, 32), 0), ((64, 64, 8), 0), ((48, 48, 6), 1),
-                                        ((128, 128, 4), 1), ((64, 64, 1), 0), ((16, 16, 24), 1)])
+                                        ((128, 128, 4), 1), ((128, 128, 4), 0), ((64, 64, 1), 0),
+                                        ((16, 16, 24), 1), ((16, 16, 24), 0)])
 def test_xy_plane_kernel(pkg, orc, shape, prec):
-    """The fused xy-plane march equals the separate-pass march and the oracle."""
+    """Both xy-plane kernels (1: one workgroup per plane, 2: split into the
+    real and imaginary parts) equal the separate-pass march and the oracle."""
     print("case", shape, prec, flush=True)
     out = {}
-    for plane in (0, 1):
+    forms = [f for f in (0, 1, 2) if not (f == 1 and prec == 0 and shape[0] == 128)]
+    for plane in forms:
         p = _params(pkg, shape, (4.0, 4.0, 3.0), 16.0, 0.35, 0.02, prec=prec, xy_plane=plane)
         wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "gyroid", 1.2, 0.2, 51)
         with pkg.Solver(p) as s:
@@ -252,7 +255,7 @@ def test_xy_plane_kernel(pkg, orc, shape, prec):
     op = _oparams(orc, p)
     pa, pb, Qo = orc.conc(op, orc.ktable(op), wA.copy(), wB.copy())
     tol = 1e-12 if prec == 0 else 2e-5
-    for plane in (0, 1):
+    for plane in forms:
         eQ, ephi = abs(out[plane][0] - Qo) / Qo, _rel(out[plane][1], pa)
         print(f"  plane={plane}: relQ={eQ:.3e} rel_phiA={ephi:.3e}")
         assert eQ < tol and ephi < tol, plane
@@ -313,3 +316,28 @@ def test_set_tiling_matches_autotune_choice(pkg):
             s.set_tiling([0, 0, 9, 0])
     print("choice", choice, flush=True)
     assert np.array_equal(fe0, fe1)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_split_plane_batched_iterations(pkg, prec):
+    """Split xy-plane kernel inside whole iterations with a batch of 3
+    instances (12 planes: a full group of 8 and a partial one): equal to the
+    separate passes (FP64 to rounding, FP32 within its own rounding)."""
+    print("case", prec, flush=True)
+    res = []
+    for plane in (0, 2):
+        # Anderson in FP64 (the least squares amplifies FP32 rounding differences between the two kernels'
+        # summation orders to ~1e-5 within a few iterations), simple mixing in FP32
+        p = _params(pkg, (32, 32, 4), (4.0, 4.0, 1.5), 15.0, 0.4, 0.02, prec=prec, xy_plane=plane, batch=3,
+                    mixing=1 - prec, anderson_m=3, anderson_mix=0.2)
+        N = 32 * 32 * 4
+        wA = np.concatenate([pkg.fields.seeded_fields(32, 32, 4, 4.0, 4.0, 1.5, "lamellar", 1.0, 0.2, 70 + b)[0]
+                             for b in range(3)])
+        wB = -wA
+        with pkg.Solver(p) as s:
+            s.set_fields(wA.reshape(3, N), wB.reshape(3, N))
+            _, fe = s.iterate(4)
+            res.append((fe.copy(), s.get_fields()[0].copy()))
+    tol = 1e-12 if prec == 0 else 2e-5
+    assert np.max(np.abs(res[0][0] - res[1][0])) < tol
+    assert _rel(res[1][1], res[0][1]) < tol
