@@ -38,12 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload_params(pkg, name, prec):
+def workload_params(pkg, name, prec, **kw):
     w = pkg.fields.WORKLOADS[name]
     n = w["n"]
     return pkg.default_params(nx=n, ny=n, nz=n, lx=w["L"], ly=w["L"], lz=w["L"], chiN=w["chiN"],
                               fA=w["fA"], ds=w["ds"], precision=prec, mixing=pkg.MIX_ANDERSON,
-                              anderson_m=w["anderson_m"], anderson_mix=0.1)
+                              anderson_m=w["anderson_m"], anderson_mix=0.1, **kw)
 
 
 def iter_bytes(N, Ns, b):
@@ -51,8 +51,8 @@ def iter_bytes(N, Ns, b):
     return N * (b * (6 * Ns + 2 * (Ns + 2)) + 64)
 
 
-def run_form(pkg, name, prec, steps, warmup, choice=None):
-    p = workload_params(pkg, name, prec)
+def run_form(pkg, name, prec, steps, warmup, choice=None, **kw):
+    p = workload_params(pkg, name, prec, **kw)
     wA, wB = pkg.fields.workload_fields(name)
     s = pkg.Solver(p)
     if choice is None:
@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--choice", default=None,
                     help="with --profile: tiling 'x,y,z,plane' to apply instead of autotuning "
                          "(a profiler distorts the autotuner's timings)")
+    ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
+                    help="with --profile: step-boundary kernels on (1, the default) or off")
     ap.add_argument("--calib-copy", action="store_true",
                     help="with --profile: also copy 64 MiB -> 64 MiB three times (PMC byte calibration)")
     a = ap.parse_args()
@@ -188,7 +190,7 @@ def main():
         nm = {"c2": "c2_cylinder_64", "c3": "c3_gyroid_128"}[wl]
         prec = pkg.FP32 if form == "fp32" else pkg.FP64
         choice = [int(c) for c in a.choice.split(",")] if a.choice else None
-        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice)
+        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice, fuse_steps=a.fuse)
         dom, passes, bpl, ach = pass_roofline(s, prec)
         print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "ms_per_iter": ms, "iters_per_s": 1e3 / ms,
                           "F": fe[0], "pass_ms": passes, "dominant": dom}))

@@ -78,12 +78,15 @@ typedef struct {
     int checkpoint;       /* 0: keep every q(s) slice; 1: sqrt(Ns) checkpointing -- keep
                              z(kK), K = ceil(sqrt(Ns+1)), recompute segments (about 2x the
                              propagator steps, O(sqrt Ns) slices of memory) */
+    int fuse_steps;       /* 1: the packed march alternates its axis order and runs each
+                             step boundary (the last axis of step t, the first of t+1) as
+                             one kernel; 0: every step's passes separately */
 } scft_params;
 
 /* Fill p with the reference's defaults (ab.txt sample sizes aside):
    FP64, simple mixing, wopt 0.05, wcmp 0.10, pi = SCFT_REFERENCE_PI, batch 1,
    Anderson m 10 / mix 0.1 (used only with SCFT_MIX_ANDERSON), HIP graph on,
-   xy_plane auto. */
+   xy_plane auto, step-boundary fusion on. */
 void scft_default_params(scft_params *p);
 
 /* NsA and NsB exactly as the reference truncates them (init_cuda.cu:330-331). Host only. */

@@ -77,7 +77,7 @@ class Params(ctypes.Structure):
         ("wopt", ctypes.c_double), ("wcmp", ctypes.c_double),
         ("anderson_m", ctypes.c_int), ("anderson_mix", ctypes.c_double),
         ("pi", ctypes.c_double), ("use_graph", ctypes.c_int), ("xy_plane", ctypes.c_int),
-        ("tiling", ctypes.c_int), ("checkpoint", ctypes.c_int),
+        ("tiling", ctypes.c_int), ("checkpoint", ctypes.c_int), ("fuse_steps", ctypes.c_int),
     ]
 
     def replace(self, **kw):

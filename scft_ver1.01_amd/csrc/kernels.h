@@ -42,6 +42,8 @@ struct PassArgs {
     C *dst;
     const T *W;          // the four W arrays
     unsigned wo[4];      // step == 0: element offsets of the four factors in W
+    C *slot;             // step-boundary kernels: z(step) goes here, dst gets step + 1 opened
+    unsigned wn[2];      // step-boundary kernels, step == 0: pre factors (real, imag) of step + 1
     const int2 *nsab;    // per-instance mode: (NsA, NsB) per instance
     const C *tw;         // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
     const typename kfac<T>::type *kf;  // n entries per instance (kf_stride apart, 0 = shared)
@@ -70,6 +72,19 @@ template <typename T>
 hipError_t launch_plane(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
                         const typename kfac<T>::type *kfy, long nplanes,
                         hipStream_t s);
+// Step-boundary pass (the march alternates its axis order, so the last axis
+// of step t is the first of step t+1): convolve along `axis` (step t), store
+// z(t) = result * post W(t) to a.slot, multiply by pre W(t+1), convolve again
+// (step t+1) and store to a.dst.  a.pre multiplies the load by pre W(t) (the
+// first step).  Per-instance mode: lines of instances whose chain ends at t
+// store only z(t); lines of finished instances are left untouched.
+template <typename T>
+hipError_t launch_fused(int axis, int n, const PassArgs<T> &a, hipStream_t s);
+// the same for the two xy-plane kernels (form 1: one workgroup per plane,
+// form 2: split into the real and imaginary parts)
+template <typename T>
+hipError_t launch_plane_fused(int form, int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
+                              const typename kfac<T>::type *kfy, long nplanes, hipStream_t s);
 // true if the split xy-plane kernel (one workgroup per plane and real part)
 // exists for nx = ny = n in precision T
 template <typename T>

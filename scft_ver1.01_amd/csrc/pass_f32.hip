@@ -9,4 +9,7 @@ template hipError_t launch_plane<float>(int, const PassArgs<float> &, const cplx
 template bool split_supported<float>(int);
 template hipError_t launch_split<float>(int, const PassArgs<float> &, const cplx<float>::type *, const float *, long,
                                         hipStream_t);
+template hipError_t launch_fused<float>(int, int, const PassArgs<float> &, hipStream_t);
+template hipError_t launch_plane_fused<float>(int, int, const PassArgs<float> &, const cplx<float>::type *, const float *,
+                                              long, hipStream_t);
 }  // namespace scft

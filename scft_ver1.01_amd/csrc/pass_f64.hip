@@ -40,4 +40,7 @@ void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *
 template bool split_supported<double>(int);
 template hipError_t launch_split<double>(int, const PassArgs<double> &, const cplx<double>::type *, const double *, long,
                                         hipStream_t);
+template hipError_t launch_fused<double>(int, int, const PassArgs<double> &, hipStream_t);
+template hipError_t launch_plane_fused<double>(int, int, const PassArgs<double> &, const cplx<double>::type *, const double *,
+                                              long, hipStream_t);
 }  // namespace scft

@@ -811,6 +811,7 @@ extern "C" void scft_default_params(scft_params *p)
     p->pi = SCFT_REFERENCE_PI;
     p->use_graph = 1;
     p->xy_plane = -1;
+    p->fuse_steps = 1;
     p->tiling = -1;
     p->checkpoint = 0;
 }
@@ -1232,52 +1233,90 @@ static inline void step_blocks(const scft_ctx *c, int t, int &br, int &bi)
 // chain (the kernels pick each instance's blocks, and leave instances whose
 // chain is shorter than t alone); t == 0: the real part uses block br's W and
 // the imaginary part block bi's for every instance.
+// arguments of one pass along `axis` (3: the xy-plane kernel, x tables)
+template <typename T>
+static PassArgs<T> pass_args(scft_ctx *c, int axis, const typename cplx<T>::type *src, typename cplx<T>::type *dst,
+                             int t, int br, int bi, bool pre, bool post)
+{
+    using C = typename cplx<T>::type;
+    const size_t BN = (size_t)c->batch * c->N;
+    const int a = axis == 3 ? 0 : axis;
+    PassArgs<T> pa;
+    pa.src = src;
+    pa.dst = dst;
+    pa.W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
+    pa.wo[0] = (unsigned)(br * BN);
+    pa.wo[1] = (unsigned)(bi * BN);
+    pa.wo[2] = (unsigned)((2 + br) * BN);
+    pa.wo[3] = (unsigned)((2 + bi) * BN);
+    pa.slot = nullptr;
+    pa.wn[0] = pa.wn[1] = 0u;
+    pa.nsab = (const int2 *)c->nsab;
+    pa.step = t;
+    pa.BN = (unsigned)BN;
+    pa.tw = (const C *)c->tw[a];
+    pa.kf = (const typename kfac<T>::type *)c->kf[a];
+    pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[a] : 0u;
+    pa.nlines = (unsigned)(c->batch * (c->N / c->n[a]));
+    pa.N = (unsigned)c->N;
+    pa.nx = c->n[0];
+    pa.ny = c->n[1];
+    pa.nz = c->n[2];
+    pa.pre = pre;
+    pa.post = post;
+    return pa;
+}
+
+// launch one unit of a step: an axis pass (0..2) or the xy-plane kernel (3);
+// `fused`: the step-boundary form
+template <typename T>
+static hipError_t launch_unit(scft_ctx *c, int unit, const PassArgs<T> &pa, bool fused)
+{
+    using C = typename cplx<T>::type;
+    if (unit == 3) {
+        const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
+        const long np = (long)c->batch * c->n[2];
+        const C *twy = (const C *)c->tw[1];
+        if (fused) return launch_plane_fused<T>(c->use_plane, c->n[0], pa, twy, kfy, np, c->stream);
+        return c->use_plane == 2 ? launch_split<T>(c->n[0], pa, twy, kfy, np, c->stream)
+                                 : launch_plane<T>(c->n[0], pa, twy, kfy, np, c->stream);
+    }
+    if (fused) return launch_fused<T>(unit, c->n[unit], pa, c->stream);
+    return launch_pass<T>(unit, c->n[unit], c->pass_variant[unit], pa, c->stream);
+}
+
+// the units of a step in forward order (the xy-plane kernel counts as one)
+static int step_units(const scft_ctx *c, int *units)
+{
+    int nu = 0;
+    if (c->use_plane) {
+        units[nu++] = 3;
+        if (c->active[2]) units[nu++] = 2;
+    } else {
+        for (int a = 0; a < 3; ++a)
+            if (c->active[a]) units[nu++] = a;
+    }
+    return nu;
+}
+
+// one packed step dst = S(src): three axis passes (or the fused xy-plane
+// kernel and the z pass).  t > 0: contour step t of every instance's packed
+// chain (the kernels pick each instance's blocks, and leave instances whose
+// chain is shorter than t alone); t == 0: the real part uses block br's W and
+// the imaginary part block bi's for every instance.
 template <typename T>
 static hipError_t enqueue_step(scft_ctx *c, const typename cplx<T>::type *src, typename cplx<T>::type *dst, int t,
                                int br, int bi)
 {
     using C = typename cplx<T>::type;
-    const size_t BN = (size_t)c->batch * c->N;
     C *work = (C *)c->work;
-    const T *W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
-    auto args = [&](int a, bool first, bool last) {
-        PassArgs<T> pa;
-        pa.src = first ? src : work;
-        pa.dst = last ? dst : work;
-        pa.W = W;
-        pa.wo[0] = (unsigned)(br * BN);
-        pa.wo[1] = (unsigned)(bi * BN);
-        pa.wo[2] = (unsigned)((2 + br) * BN);
-        pa.wo[3] = (unsigned)((2 + bi) * BN);
-        pa.nsab = (const int2 *)c->nsab;
-        pa.step = t;
-        pa.BN = (unsigned)BN;
-        pa.tw = (const C *)c->tw[a];
-        pa.kf = (const typename kfac<T>::type *)c->kf[a];
-        pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[a] : 0u;
-        pa.nlines = (unsigned)(c->batch * (c->N / c->n[a]));
-        pa.N = (unsigned)c->N;
-        pa.nx = c->n[0];
-        pa.ny = c->n[1];
-        pa.nz = c->n[2];
-        pa.pre = first;
-        pa.post = last;
-        return pa;
-    };
-    if (c->use_plane) {
-        const bool zact = c->active[2];
-        const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
-        const long np = (long)c->batch * c->n[2];
-        hipError_t e = c->use_plane == 2
-                           ? launch_split<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], kfy, np, c->stream)
-                           : launch_plane<T>(c->n[0], args(0, true, !zact), (const C *)c->tw[1], kfy, np, c->stream);
-        if (e != hipSuccess || !zact) return e;
-        return launch_pass<T>(2, c->n[2], c->pass_variant[2], args(2, false, true), c->stream);
-    }
-    for (int a = 0; a < 3; ++a) {
-        if (!c->active[a]) continue;
-        hipError_t e = launch_pass<T>(a, c->n[a], c->pass_variant[a], args(a, a == c->first_axis, a == c->last_axis),
-                                      c->stream);
+    int units[3];
+    const int nu = step_units(c, units);
+    for (int k = 0; k < nu; ++k) {
+        const bool first = k == 0, last = k == nu - 1;
+        hipError_t e = launch_unit<T>(
+            c, units[k], pass_args<T>(c, units[k], first ? src : work, last ? dst : work, t, br, bi, first, last),
+            false);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1289,6 +1328,52 @@ static typename cplx<T>::type *slot_ptr(scft_ctx *c, long k)
     return (typename cplx<T>::type *)c->slots + (size_t)k * c->batch * c->N;
 }
 
+// destination of z(t) in the slice store (checkpoint mode keeps z(kK) and
+// two scratch slices)
+template <typename T>
+static typename cplx<T>::type *march_slot(scft_ctx *c, int t)
+{
+    if (!c->ckpt) return slot_ptr<T>(c, t);
+    if (t % c->ck_K == 0) return slot_ptr<T>(c, t / c->ck_K);
+    return slot_ptr<T>(c, c->ck_n + c->ck_K + (t & 1));
+}
+
+// The packed march with step-boundary kernels.  S = W C_x C_y C_z W and the
+// axis convolutions commute, so odd steps run the units in forward order and
+// even steps in reverse: the last unit of step t is the first of step t+1, and
+// one kernel closes t (z(t) to its slot) and opens t+1 on the data in
+// registers.  Steps then cost nu - 1 launches instead of nu (nu >= 2 units).
+template <typename T>
+static hipError_t enqueue_march_fused(scft_ctx *c, int nsteps, typename cplx<T>::type **last)
+{
+    using C = typename cplx<T>::type;
+    C *work = (C *)c->work;
+    const C *slot0 = slot_ptr<T>(c, 0);
+    int units[3];
+    const int nu = step_units(c, units);
+    C *dst = nullptr;
+    for (int t = 1; t <= nsteps; ++t) {
+        dst = march_slot<T>(c, t);
+        for (int k = 0; k < nu; ++k) {
+            const int u = (t & 1) ? units[k] : units[nu - 1 - k];
+            const bool first = k == 0, lastu = k == nu - 1;
+            if (first && t > 1) continue;  // opened by step t-1's boundary kernel
+            const C *src = (first && t == 1) ? slot0 : work;
+            hipError_t e;
+            if (lastu && t < nsteps) {
+                PassArgs<T> pa = pass_args<T>(c, u, src, work, t, 0, 0, first, false);
+                pa.slot = dst;
+                e = launch_unit<T>(c, u, pa, true);
+            } else {
+                e = launch_unit<T>(c, u, pass_args<T>(c, u, src, lastu ? dst : work, t, 0, 0, first, lastu), false);
+            }
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (last) *last = dst;
+    return hipSuccess;
+}
+
 // march `nsteps` steps from slot 0.  block -1: the packed chain; 0/1: one
 // block's W on both parts (sovDifFft's march, full mode only).  In checkpoint
 // mode only z(kK) and the last slice are kept; *last receives z(nsteps).
@@ -1296,12 +1381,12 @@ template <typename T>
 static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block, typename cplx<T>::type **last = nullptr)
 {
     using C = typename cplx<T>::type;
+    int units[3];
+    if (block < 0 && c->p.fuse_steps && nsteps > 1 && step_units(c, units) >= 2)
+        return enqueue_march_fused<T>(c, nsteps, last);
     C *cur = slot_ptr<T>(c, 0);
     for (int t = 1; t <= nsteps; ++t) {
-        C *dst;
-        if (!c->ckpt) dst = slot_ptr<T>(c, t);
-        else if (t % c->ck_K == 0) dst = slot_ptr<T>(c, t / c->ck_K);
-        else dst = slot_ptr<T>(c, c->ck_n + c->ck_K + (t & 1));
+        C *dst = march_slot<T>(c, t);
         hipError_t e = block < 0 ? enqueue_step<T>(c, cur, dst, t, 0, 0) : enqueue_step<T>(c, cur, dst, 0, block, block);
         if (e != hipSuccess) return e;
         cur = dst;
@@ -1647,6 +1732,8 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.dst = (C *)c->work;
     pa.W = (const T *)c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
+    pa.slot = nullptr;
+    pa.wn[0] = pa.wn[1] = 0u;
     pa.nsab = (const int2 *)c->nsab;
     pa.step = 0;
     pa.BN = (unsigned)(c->batch * c->N);
@@ -1670,6 +1757,50 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RT) k_fill(T *__restrict__ p, T v, long n)
+{
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) p[i] = v;
+}
+
+// mean duration (ms) of `reps` packed marches of Ns_max steps (the propagator
+// part of an iteration, in the current plane form and tilings) on stand-in
+// factors W = 1, W/N = 1/N and z(0) = 1.  Overwrites the W factors and the
+// slice store, which the next iteration or concentrations call recomputes.
+static int time_march(scft_ctx *ctx, int reps, float *ms)
+{
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    const double invN = 1.0 / (double)ctx->N;
+    if (ctx->fp32) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz, 1.0f,
+                           (long)(2 * BN));
+        hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz + 2 * BN,
+                           (float)invN, (long)(2 * BN));
+        hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
+                           (const double *)nullptr, (long)BN);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (double *)ctx->wdz, 1.0,
+                           (long)(2 * BN));
+        hipLaunchKernelGGL(k_fill<double>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream,
+                           (double *)ctx->wdz + 2 * BN, invN, (long)(2 * BN));
+        hipLaunchKernelGGL(k_seed_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (double2 *)ctx->slots,
+                           (const double *)nullptr, (long)BN);
+    }
+    HIPX(ctx, hipGetLastError());
+    auto march = [&]() {
+        return ctx->fp32 ? enqueue_march<float>(ctx, ctx->Ns_max, -1) : enqueue_march<double>(ctx, ctx->Ns_max, -1);
+    };
+    HIPX(ctx, march());  // untimed: code loading, first touch
+    HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) HIPX(ctx, march());
+    HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPX(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    HIPX(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t / reps;
+    return SCFT_OK;
 }
 
 // mean duration (ms) of `reps` back-to-back launches of one pass kernel on a
@@ -1739,23 +1870,29 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
         per_axis[a] = bt;
         best_sep += bt;
     }
-    // xy-plane forms: forced by p.xy_plane > 0, off at 0, timed against the
-    // separate x and y passes at -1
+    // xy-plane form: forced by p.xy_plane > 0, off at 0; at -1 every form that
+    // fits is timed over whole marches (with step-boundary fusion the plane and
+    // line kernels trade places differently than their single-pass times say)
+    (void)best_sep;
     int plane = ctx->p.xy_plane > 0 ? ctx->p.xy_plane : 0;
-    float tplane = 0.0f;
-    const float tz = ctx->active[2] ? per_axis[2] : 0.0f;
-    float best = best_sep;
-    for (int form = 1; form <= 2 && ctx->p.xy_plane != 0; ++form) {
-        if (!plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form)) continue;
-        if (ctx->p.xy_plane > 0 && form != ctx->p.xy_plane) continue;
-        float t = 0.0f;
-        int rc = time_pass_variant(ctx, 2 + form, reps, 0, &t);
-        if (rc) return rc;
-        if (ctx->p.xy_plane > 0 || t + tz < best) {
-            best = t + tz;
-            plane = form;
-            tplane = t;
+    if (ctx->p.xy_plane < 0) {
+        float best = 0.0f;
+        for (int form = 0; form <= 2; ++form) {
+            if (form > 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form)) continue;
+            ctx->use_plane = form;
+            float t = 0.0f;
+            int rc = time_march(ctx, 2, &t);
+            if (rc) return rc;
+            if (form == 0 || t < best) {
+                best = t;
+                plane = form;
+            }
         }
+    }
+    float tplane = 0.0f;
+    if (plane) {
+        int rc = time_pass_variant(ctx, 2 + plane, reps, 0, &tplane);
+        if (rc) return rc;
     }
     ctx->use_plane = plane;
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);

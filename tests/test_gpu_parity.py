@@ -341,3 +341,36 @@ def test_split_plane_batched_iterations(pkg, prec):
     tol = 1e-12 if prec == 0 else 2e-5
     assert np.max(np.abs(res[0][0] - res[1][0])) < tol
     assert _rel(res[1][1], res[0][1]) < tol
+
+
+@pytest.mark.parametrize("shape,plane,prec,ckpt", [
+    ((32, 32, 32), 0, 0, 0), ((32, 32, 32), 1, 0, 0), ((32, 32, 32), 2, 0, 0), ((16, 24, 8), 0, 0, 0),
+    ((24, 20, 1), 0, 0, 0), ((16, 16, 1), 2, 0, 0), ((32, 32, 32), 2, 1, 0), ((32, 32, 32), 0, 0, 1),
+    ((64, 64, 16), 1, 1, 1)])
+def test_fused_march_matches_unfused(pkg, orc, shape, plane, prec, ckpt):
+    """The march with step-boundary kernels (alternating axis order, one
+    kernel per boundary) equals the step-by-step march: FP64 to rounding (the
+    axis order changes the summation order), FP32 within its rounding, and
+    both the oracle in FP64."""
+    print("case", shape, plane, prec, ckpt, flush=True)
+    res = {}
+    for fuse in (0, 1):
+        p = _params(pkg, shape, (4.0, 3.6, 3.2), 16.0, 0.35, 0.02, prec=prec, xy_plane=plane, checkpoint=ckpt,
+                    fuse_steps=fuse, batch=2)
+        N = p.nx * p.ny * p.nz
+        wA = np.stack([pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "gyroid", 1.2, 0.2, 80 + b)[0]
+                       for b in range(2)])
+        wB = -0.5 * wA
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations().copy()
+            phiA = s.get_fields()[0].copy()
+        res[fuse] = (Q, phiA)
+    tol = 1e-12 if prec == 0 else 2e-5
+    assert np.max(np.abs(res[1][0] - res[0][0]) / res[0][0]) < tol
+    assert _rel(res[1][1], res[0][1]) < tol
+    if prec == 0:
+        op = _oparams(orc, p)
+        pa, pb, Qo = orc.conc(op, orc.ktable(op), wA[1].copy(), wB[1].copy())
+        assert abs(res[1][0][1] - Qo) / Qo < 1e-12
+        assert _rel(res[1][1][1], pa) < 1e-12

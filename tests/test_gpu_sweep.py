@@ -32,7 +32,9 @@ def test_sweep_matches_single_instances(pkg, mixing, prec):
             s.set_fields(*fields[i])
             _, fe1 = s.iterate(6)
             phi1 = s.get_fields()[0][0]
-        tol = 1e-12 if prec == 0 else 1e-6
+        # the sweep's autotuned kernels (plane forms, tilings) sum in another order than the single
+        # instance's defaults: FP64 rounding differences, grown by 6 iterations (Anderson's least squares)
+        tol = 1e-11 if prec == 0 else 1e-6
         assert np.max(np.abs(fe[i] - fe1[0])) < tol * max(1, np.max(np.abs(fe1[0]))), (i, fe[i], fe1[0])
         assert _rel(phis[i], phi1) < tol, i
 
