@@ -93,24 +93,42 @@ def single_eval_error(pkg, name, wA, wB):
             "max_dphiA": float(np.max(np.abs(a32 - a64))), "max_dphiB": float(np.max(np.abs(b32 - b64)))}
 
 
+KERNEL_NAMES = {"x": "k_line_conv x pass", "y": "k_line_conv y pass", "z": "k_line_conv z pass",
+                "xy_plane": "k_plane_xy", "xy_split": "k_plane_split",
+                "fused_x": "k_line_fused x boundary", "fused_y": "k_line_fused y boundary",
+                "fused_z": "k_line_fused z boundary", "fused_plane": "k_plane_xy boundary form",
+                "fused_split": "k_plane_split boundary form"}
+CODES = {"x": 0, "y": 1, "z": 2, "xy_plane": 3, "xy_split": 4, "fused_x": 5, "fused_y": 6, "fused_z": 7,
+         "fused_plane": 8, "fused_split": 9}
+
+
 def pass_roofline(s, prec, reps=50):
-    """Mean duration of each axis pass (HIP events around `reps` back-to-back
-    launches on the solver stream) and the dominant one's bandwidth."""
+    """Per-launch duration (HIP events around `reps` back-to-back launches on
+    the solver stream) of every kernel the fused march of this solver's tiling
+    runs, each kernel's share of a step, and the dominant kernel's bandwidth.
+
+    Steady-state step of the fused march (all three axes active):
+      line form  : one middle y pass + one boundary kernel (x|x and z|z alternate)
+      plane forms: one boundary kernel (plane|plane and z|z alternate)
+    Algorithmic bytes per launch: a pass reads and writes the complex field
+    (4bN); a boundary kernel reads the field, writes z(t) and the opened field
+    (6bN) and reads the W factors (2bN when post = pre/N exactly, else 4bN)."""
     b = 4 if prec else 8
-    out = {}
-    for axis, nm in enumerate("xyz"):
-        out[nm] = s.time_pass(axis, reps)
-    for form, nm in ((3, "xy_plane"), (4, "xy_split")):
-        try:
-            out[nm] = s.time_pass(form, reps)
-        except Exception:
-            pass
     plane = getattr(s, "tuning", ([0, 0, 0, 0], None))[0][3]
-    used = {k: v for k, v in out.items() if k in {0: ("x", "y", "z"), 1: ("xy_plane", "z"), 2: ("xy_split", "z")}[plane]}
-    dom = max(used, key=used.get)
-    bytes_per_launch = 4 * b * s.N  # read + write of the complex field (2b bytes per point each way)
-    achieved = bytes_per_launch / (out[dom] * 1e-3) / 1e9
-    return dom, out, bytes_per_launch, achieved
+    N = s.N
+    n = round(N ** (1 / 3))
+    post_exact = (N & (N - 1)) == 0
+    if plane == 0:
+        kernels = {"y": 1.0, "fused_x": 0.5, "fused_z": 0.5}
+    else:
+        kernels = {("fused_plane" if plane == 1 else "fused_split"): 0.5, "fused_z": 0.5}
+    times = {k: s.time_pass(CODES[k], reps) for k in kernels}
+    share = {k: kernels[k] * times[k] for k in kernels}
+    dom = max(share, key=share.get)
+    bpl = {k: (4 * b * N if not k.startswith("fused") else (8 if post_exact else 10) * b * N) for k in kernels}
+    achieved = bpl[dom] / (times[dom] * 1e-3) / 1e9
+    return {"dominant": dom, "kernel": KERNEL_NAMES[dom], "ms_per_launch": times, "step_share_ms": share,
+            "bytes_per_launch": bpl, "achieved": achieved, "grid": n}
 
 
 def calibration_copy(nbytes, reps):
@@ -174,6 +192,8 @@ def main():
     ap.add_argument("--choice", default=None,
                     help="with --profile: tiling 'x,y,z,plane' to apply instead of autotuning "
                          "(a profiler distorts the autotuner's timings)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="with --profile and --choice: only the roofline leg (per-kernel launches, no iterations)")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
                     help="with --profile: step-boundary kernels on (1, the default) or off")
     ap.add_argument("--calib-copy", action="store_true",
@@ -190,10 +210,24 @@ def main():
         nm = {"c2": "c2_cylinder_64", "c3": "c3_gyroid_128"}[wl]
         prec = pkg.FP32 if form == "fp32" else pkg.FP64
         choice = [int(c) for c in a.choice.split(",")] if a.choice else None
+        if a.roofline_only:
+            if choice is None:
+                sys.exit("--roofline-only needs --choice")
+            p = workload_params(pkg, nm, prec, fuse_steps=a.fuse)
+            with pkg.Solver(p) as s:
+                s.set_tiling(choice)
+                s.tuning = (choice, None)
+                wA, wB = pkg.fields.workload_fields(nm)
+                s.set_fields(wA, wB)
+                s.concentrations()  # W factors of the workload's fields
+                print(json.dumps({"profile": a.profile, "tiling": choice, "roofline": pass_roofline(s, prec)}))
+            if a.calib_copy:
+                calibration_copy(64 << 20, 3)
+            return
         s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice, fuse_steps=a.fuse)
-        dom, passes, bpl, ach = pass_roofline(s, prec)
+        rl = pass_roofline(s, prec)
         print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "ms_per_iter": ms, "iters_per_s": 1e3 / ms,
-                          "F": fe[0], "pass_ms": passes, "dominant": dom}))
+                          "F": fe[0], "roofline": rl}))
         s.close()
         if a.calib_copy:
             calibration_copy(64 << 20, 3)
@@ -202,8 +236,9 @@ def main():
     s32, ms32, fe32, phi32 = run_form(pkg, name, pkg.FP32, a.steps, a.warmup)
     s64, ms64, fe64, phi64 = run_form(pkg, name, pkg.FP64, a.steps, a.warmup)
     Ns = s32.NsA + s32.NsB
-    dom, passes32, bpl, ach = pass_roofline(s32, 1)
-    _, passes64, bpl64, ach64 = pass_roofline(s64, 0)
+    rl32 = pass_roofline(s32, 1)
+    rl64 = pass_roofline(s64, 0)
+    dom = rl32["dominant"]
     N = s32.N
     it32 = 1e3 / ms32
     it64 = 1e3 / ms64
@@ -215,7 +250,7 @@ def main():
             "trajectory_dF_fp32_vs_fp64": abs(fe32[0] - fe64[0]),
             "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
             "F_fp64": fe64[0], "iterations_compared": a.warmup + a.steps,
-            "pass_ms_fp32": passes32, "pass_ms_fp64": passes64,
+            "kernels_fp32": rl32, "kernels_fp64": rl64,
         }
     }
     _, _, wa64, wb64 = s64.get_fields()
@@ -226,8 +261,8 @@ def main():
         name3 = "c3_gyroid_128"
         t32, m32, f32, p32 = run_form(pkg, name3, pkg.FP32, max(5, a.steps // 2), 3)
         t64, m64, f64, p64 = run_form(pkg, name3, pkg.FP64, max(5, a.steps // 2), 3)
-        _, pass3_32, _, _ = pass_roofline(t32, 1, reps=20)
-        _, pass3_64, _, _ = pass_roofline(t64, 0, reps=20)
+        rl3_32 = pass_roofline(t32, 1, reps=20)
+        rl3_64 = pass_roofline(t64, 0, reps=20)
         _, _, wa64, wb64 = t64.get_fields()
         N3 = t32.N
         Ns3 = t32.NsA + t32.NsB
@@ -237,7 +272,7 @@ def main():
             "fp64_iter_hbm_frac": iter_bytes(N3, Ns3, 8) * (1e3 / m64) / 1e9 / HBM_PEAK_GBPS,
             "trajectory_dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
             "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
-            "pass_ms_fp32": pass3_32, "pass_ms_fp64": pass3_64,
+            "kernels_fp32": rl3_32, "kernels_fp64": rl3_64,
         }
         t32.close()
         t64.close()
@@ -272,8 +307,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        key = f"c2_fp32:{dom}"
-        traffic = tr.get(key)
+        traffic = tr.get(f"c2_fp32:{dom}")
     out = {
         "metric": "SCFT iters/s at 64^3,128^3 Ns=200 (FP32 vs FP64 form), %HBM peak, dF vs FP64",
         "value": it32,
@@ -291,10 +325,10 @@ def main():
                                "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
                    "batch": 1},
         "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0]},
-        "roofline": {"bound": "hbm", "kernel": {"xy_plane": "k_plane_xy (fp32)", "xy_split": "k_plane_split (fp32)"}.get(
-                         dom, f"k_line_conv {dom}-pass (fp32)"), "achieved": ach,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS,
-                     "traffic": traffic, "bytes_per_launch": bpl, "ms_per_launch": passes32[dom]},
+        "roofline": {"bound": "hbm", "kernel": rl32["kernel"] + " (fp32, 64^3)", "achieved": rl32["achieved"],
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": rl32["achieved"] / HBM_PEAK_GBPS,
+                     "traffic": traffic, "bytes_per_launch": rl32["bytes_per_launch"][dom],
+                     "ms_per_launch": rl32["ms_per_launch"][dom]},
         "cpu_baseline": cpu,
         "forms": forms,
     }

@@ -137,7 +137,8 @@ int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
 
 /* Kernel-level timing for the roofline: runs `reps` back-to-back launches of
    the propagator axis pass `axis` (0 x, 1 y, 2 z, 3 the fused xy-plane kernel,
-   4 the split xy-plane kernel)
+   4 the split xy-plane kernel; 5, 6, 7 the step-boundary x, y, z kernels and
+   8, 9 the step-boundary forms of 3 and 4)
    on a copy of slice 0
    between two HIP events and returns the mean duration in ms.  Leaves the
    solver state unchanged except the scratch field. */

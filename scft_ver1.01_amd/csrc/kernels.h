@@ -44,6 +44,8 @@ struct PassArgs {
     unsigned wo[4];      // step == 0: element offsets of the four factors in W
     C *slot;             // step-boundary kernels: z(step) goes here, dst gets step + 1 opened
     unsigned wn[2];      // step-boundary kernels, step == 0: pre factors (real, imag) of step + 1
+    T post_scale;        // step-boundary kernels with post_from_pre: post factor = pre factor * post_scale
+    int post_from_pre;   //   (exact: 1/N a power of two and no folded length-1 axis)
     const int2 *nsab;    // per-instance mode: (NsA, NsB) per instance
     const C *tw;         // n entries, (cos 2 pi k/n, -sin 2 pi k/n)
     const typename kfac<T>::type *kf;  // n entries per instance (kf_stride apart, 0 = shared)

@@ -729,6 +729,7 @@ struct scft_ctx {
     int2 *nsab = nullptr;      // device (NsA, NsB) per instance
     double *chiN_b = nullptr;  // device chiN per instance
     bool kf_per_inst = false;  // one k table per instance (per-instance box) or one shared
+    int post_exact = 0;        // post W factors equal pre / N exactly (see upload_tables)
     double *post_b = nullptr;  // device per-instance post-step factor (1/N x length-1 axis factors)
     int Ns_max = 0;
     std::vector<int2> nsab_h;  // host copy of nsab
@@ -893,6 +894,9 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
     }
     c->post = post[0];
     c->kf_per_inst = inst_L != nullptr;
+    // post factor = pre factor / N bit for bit when 1/N is a power of two and no axis is folded in
+    c->post_exact = (c->N & (c->N - 1)) == 0 &&
+                    std::all_of(post.begin(), post.end(), [&](double v) { return v == 1.0 / (double)c->N; });
     int rc;
     if ((rc = dalloc(c, (void **)&c->post_b, sizeof(double) * c->batch))) return rc;
     HIPX(c, hipMemcpy(c->post_b, post.data(), sizeof(double) * c->batch, hipMemcpyHostToDevice));
@@ -1251,6 +1255,8 @@ static PassArgs<T> pass_args(scft_ctx *c, int axis, const typename cplx<T>::type
     pa.wo[3] = (unsigned)((2 + bi) * BN);
     pa.slot = nullptr;
     pa.wn[0] = pa.wn[1] = 0u;
+    pa.post_scale = (T)(1.0 / (double)c->N);
+    pa.post_from_pre = c->post_exact;
     pa.nsab = (const int2 *)c->nsab;
     pa.step = t;
     pa.BN = (unsigned)BN;
@@ -1732,12 +1738,15 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.dst = (C *)c->work;
     pa.W = (const T *)c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
-    pa.slot = nullptr;
+    // step-boundary kernels (axis 5..9) store z(t) to a scratch slice
+    pa.slot = axis >= 5 ? slot_ptr<T>(c, 1) : nullptr;
     pa.wn[0] = pa.wn[1] = 0u;
+    pa.post_scale = (T)(1.0 / (double)c->N);
+    pa.post_from_pre = c->post_exact;
     pa.nsab = (const int2 *)c->nsab;
     pa.step = 0;
     pa.BN = (unsigned)(c->batch * c->N);
-    const int ax = axis >= 3 ? 0 : axis;
+    const int ax = (axis >= 5 && axis <= 7) ? axis - 5 : (axis >= 3 ? 0 : axis);
     pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[ax] : 0u;
     pa.tw = (const C *)c->tw[ax];
     pa.kf = (const typename kfac<T>::type *)c->kf[ax];
@@ -1751,8 +1760,11 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
     const long np = (long)c->batch * c->n[2];
     for (int r = 0; r < reps; ++r) {
-        hipError_t e = axis == 4   ? launch_split<T>(c->n[0], pa, (const C *)c->tw[1], kfy, np, c->stream)
-                       : axis == 3 ? launch_plane<T>(c->n[0], pa, (const C *)c->tw[1], kfy, np, c->stream)
+        const C *twy = (const C *)c->tw[1];
+        hipError_t e = axis >= 8   ? launch_plane_fused<T>(axis - 7, c->n[0], pa, twy, kfy, np, c->stream)
+                       : axis >= 5 ? launch_fused<T>(ax, c->n[ax], pa, c->stream)
+                       : axis == 4 ? launch_split<T>(c->n[0], pa, twy, kfy, np, c->stream)
+                       : axis == 3 ? launch_plane<T>(c->n[0], pa, twy, kfy, np, c->stream)
                                    : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
         if (e != hipSuccess) return e;
     }
@@ -1831,10 +1843,12 @@ static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, flo
 
 extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
-    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 4) return SCFT_E_INVALID;
-    if (axis >= 3 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], axis - 2))
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 9) return SCFT_E_INVALID;
+    const int form = axis == 3 || axis == 8 ? 1 : (axis == 4 || axis == 9 ? 2 : 0);
+    if (form && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form))
         return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel of this form for this grid");
-    if (axis < 3 && !ctx->active[axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
+    const int line_axis = axis < 3 ? axis : (axis >= 5 && axis <= 7 ? axis - 5 : -1);
+    if (line_axis >= 0 && !ctx->active[line_axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
     HIPX(ctx, hipSetDevice(ctx->device));
     return time_pass_variant(ctx, axis, reps, axis < 3 ? ctx->pass_variant[axis] : 0, ms);
 }
