@@ -35,6 +35,8 @@ struct Config {
     unsigned long long seed = 1;
     int exact_output = 0;          // 1: pha files with %.17g (bit-exact resume)
     int pi_exact = 0;              // 1: use M_PI instead of the reference's 3.1415926535
+    int autotune = 1;              // 1: time the kernel tilings and plane forms once before iterating
+    int checkpoint = 0;            // 1: sqrt(Ns) checkpointing of the slice store
 };
 
 // parse flags; returns 0, or prints the problem and returns nonzero

@@ -116,6 +116,8 @@ int read_config(const std::string &path, Config &c, std::string &err)
         else if (key == "seed") c.seed = std::strtoull(vs, nullptr, 10);
         else if (key == "exact_output") c.exact_output = std::atoi(vs);
         else if (key == "pi_exact") c.pi_exact = std::atoi(vs);
+        else if (key == "autotune") c.autotune = std::atoi(vs);
+        else if (key == "checkpoint") c.checkpoint = std::atoi(vs);
         else {
             std::fclose(fp);
             err = path + ": unknown key '" + key + "'";

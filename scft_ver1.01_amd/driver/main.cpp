@@ -123,10 +123,16 @@ int main(int argc, char **argv)
     p.anderson_m = c.anderson_m;
     p.anderson_mix = c.anderson_mix;
     if (c.pi_exact) p.pi = 3.14159265358979323846;
+    p.checkpoint = c.checkpoint;
     scft_ctx *ctx = nullptr;
     int rc = scft_create(&p, c.which_gpus[0], &ctx);
     if (rc != SCFT_OK) {
         std::fprintf(stderr, "scft_create: %s\n", scft_last_error(nullptr));
+        return 1;
+    }
+    if (c.autotune && (rc = scft_autotune(ctx, 10, nullptr, nullptr)) != SCFT_OK) {
+        std::fprintf(stderr, "scft_autotune: %s\n", scft_last_error(ctx));
+        scft_destroy(ctx);
         return 1;
     }
     if ((rc = scft_set_fields(ctx, wA.data(), wB.data())) != SCFT_OK) {

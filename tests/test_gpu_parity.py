@@ -293,8 +293,9 @@ def test_autotune_keeps_results(pkg):
             s.set_fields(wA, wB)
             _, fe = s.iterate(5)
             out.append(fe)
-    # plane fusion and tiling change the order of operations only within FP32 rounding
-    assert np.max(np.abs(out[0] - out[1])) < 2e-6
+    # plane forms, tilings and step-boundary fusion change the order of operations only within FP32
+    # rounding (~1e-7 per evaluation, over 5 iterations)
+    assert np.max(np.abs(out[0] - out[1])) < 5e-6
 
 
 @pytest.mark.gpu
