@@ -97,9 +97,10 @@ KERNEL_NAMES = {"x": "k_line_conv x pass", "y": "k_line_conv y pass", "z": "k_li
                 "xy_plane": "k_plane_xy", "xy_split": "k_plane_split",
                 "fused_x": "k_line_fused x boundary", "fused_y": "k_line_fused y boundary",
                 "fused_z": "k_line_fused z boundary", "fused_plane": "k_plane_xy boundary form",
-                "fused_split": "k_plane_split boundary form"}
+                "fused_split": "k_plane_split boundary form",
+                "fused_split3": "k_plane_split boundary form, three-level tiling"}
 CODES = {"x": 0, "y": 1, "z": 2, "xy_plane": 3, "xy_split": 4, "fused_x": 5, "fused_y": 6, "fused_z": 7,
-         "fused_plane": 8, "fused_split": 9}
+         "fused_plane": 8, "fused_split": 9, "fused_split3": 11}
 
 
 def pass_roofline(s, prec, reps=50):
@@ -121,7 +122,7 @@ def pass_roofline(s, prec, reps=50):
     if plane == 0:
         kernels = {"y": 1.0, "fused_x": 0.5, "fused_z": 0.5}
     else:
-        kernels = {("fused_plane" if plane == 1 else "fused_split"): 0.5, "fused_z": 0.5}
+        kernels = {{1: "fused_plane", 2: "fused_split", 3: "fused_split3"}[plane]: 0.5, "fused_z": 0.5}
     times = {k: s.time_pass(CODES[k], reps) for k in kernels}
     share = {k: kernels[k] * times[k] for k in kernels}
     dom = max(share, key=share.get)
@@ -273,6 +274,7 @@ def main():
             "trajectory_dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
             "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
             "kernels_fp32": rl3_32, "kernels_fp64": rl3_64,
+            "tiling_fp32": t32.tuning[0], "tiling_fp64": t64.tuning[0],
         }
         t32.close()
         t64.close()

@@ -58,43 +58,37 @@ struct PassArgs {
     int step;            // contour step (per-instance mode) or 0
 };
 
-// true if the pass kernels have an instantiation for line length n
+// true if the pass kernels have an instantiation for line length n (and the
+// three-level tiling for it)
 bool pass_supported(int n);
+bool pass3_supported(int n);
 // launch one pass along `axis` (n = length of that axis) in tiling variant
-// `variant` (0..PASS_VARIANTS-1: one or two lines per thread group)
-constexpr int PASS_VARIANTS = 2;
+// `variant`: 0 one line per thread group, 1 two lines per group, 2 the
+// three-level tiling (twice the threads per line; lengths 32, 64, 128, 256)
+constexpr int PASS_VARIANTS = 3;
 template <typename T>
 hipError_t launch_pass(int axis, int n, int variant, const PassArgs<T> &a, hipStream_t s);
-// true if an xy-plane kernel exists for nx = ny = n in precision T (LDS fit)
+// xy-plane kernels for nx = ny = n.  form 1: one workgroup per plane; form 2:
+// one workgroup per plane and real part; form 3: form 2 with the three-level
+// tiling.  nplanes = batch * nz.
 template <typename T>
-bool plane_supported(int n);
-// x and y passes of every plane (nplanes = batch * nz) in one launch; `a`
-// carries the x tables, twy/kfy the y tables
+bool plane_form_supported(int form, int n);
 template <typename T>
-hipError_t launch_plane(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
-                        const typename kfac<T>::type *kfy, long nplanes,
-                        hipStream_t s);
+hipError_t launch_plane_form(int form, int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
+                             const typename kfac<T>::type *kfy, long nplanes, hipStream_t s);
 // Step-boundary pass (the march alternates its axis order, so the last axis
 // of step t is the first of step t+1): convolve along `axis` (step t), store
 // z(t) = result * post W(t) to a.slot, multiply by pre W(t+1), convolve again
 // (step t+1) and store to a.dst.  a.pre multiplies the load by pre W(t) (the
 // first step).  Per-instance mode: lines of instances whose chain ends at t
-// store only z(t); lines of finished instances are left untouched.
+// store only z(t); lines of finished instances are left untouched.  variant 2
+// selects the three-level tiling (others the one-line two-level one).
 template <typename T>
-hipError_t launch_fused(int axis, int n, const PassArgs<T> &a, hipStream_t s);
-// the same for the two xy-plane kernels (form 1: one workgroup per plane,
-// form 2: split into the real and imaginary parts)
+hipError_t launch_fused(int axis, int n, int variant, const PassArgs<T> &a, hipStream_t s);
+// the step-boundary forms of the xy-plane kernels
 template <typename T>
 hipError_t launch_plane_fused(int form, int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
                               const typename kfac<T>::type *kfy, long nplanes, hipStream_t s);
-// true if the split xy-plane kernel (one workgroup per plane and real part)
-// exists for nx = ny = n in precision T
-template <typename T>
-bool split_supported(int n);
-// the split form of launch_plane: 2 * 8 * ceil(nplanes / 8) workgroups
-template <typename T>
-hipError_t launch_split(int n, const PassArgs<T> &a, const typename cplx<T>::type *twy,
-                        const typename kfac<T>::type *kfy, long nplanes, hipStream_t s);
 // the (A, B, lines-per-workgroup, threads) tiling used for (axis, n)
 void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *threads);
 

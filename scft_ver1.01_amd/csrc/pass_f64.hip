@@ -4,9 +4,6 @@
 namespace scft {
 
 template hipError_t launch_pass<double>(int, int, int, const PassArgs<double> &, hipStream_t);
-template bool plane_supported<double>(int);
-template hipError_t launch_plane<double>(int, const PassArgs<double> &, const cplx<double>::type *, const double *, long,
-                                        hipStream_t);
 
 bool pass_supported(int n)
 {
@@ -14,6 +11,17 @@ bool pass_supported(int n)
 #define SCFT_CASE(N_, A_, B_) \
     case N_: return true;
         SCFT_PASS_SIZES(SCFT_CASE)
+#undef SCFT_CASE
+    default: return false;
+    }
+}
+
+bool pass3_supported(int n)
+{
+    switch (n) {
+#define SCFT_CASE(N_, A_, B_) \
+    case N_: return true;
+        SCFT_PASS3_SIZES(SCFT_CASE)
 #undef SCFT_CASE
     default: return false;
     }
@@ -37,10 +45,10 @@ void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *
     }
 }
 
-template bool split_supported<double>(int);
-template hipError_t launch_split<double>(int, const PassArgs<double> &, const cplx<double>::type *, const double *, long,
-                                        hipStream_t);
-template hipError_t launch_fused<double>(int, int, const PassArgs<double> &, hipStream_t);
+template bool plane_form_supported<double>(int, int);
+template hipError_t launch_plane_form<double>(int, int, const PassArgs<double> &, const cplx<double>::type *, const double *,
+                                             long, hipStream_t);
+template hipError_t launch_fused<double>(int, int, int, const PassArgs<double> &, hipStream_t);
 template hipError_t launch_plane_fused<double>(int, int, const PassArgs<double> &, const cplx<double>::type *, const double *,
                                               long, hipStream_t);
 }  // namespace scft

@@ -937,14 +937,13 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
     return SCFT_OK;
 }
 
-// true if xy-plane form `form` (1 xy-plane, 2 split xy-plane) has a kernel for
+// true if xy-plane form `form` (1 xy-plane, 2 split xy-plane, 3 split with the
+// three-level tiling) has a kernel for
 // an nx x ny plane in this precision that fits a CU's LDS
 static bool plane_fits(bool fp32, int nx, int ny, int form)
 {
-    if (nx != ny || nx < 2) return false;
-    if (form == 1) return fp32 ? plane_supported<float>(nx) : plane_supported<double>(nx);
-    if (form == 2) return fp32 ? split_supported<float>(nx) : split_supported<double>(nx);
-    return false;
+    if (nx != ny || nx < 2 || form < 1 || form > 3) return false;
+    return fp32 ? plane_form_supported<float>(form, nx) : plane_form_supported<double>(form, nx);
 }
 
 // Context constructor.  inst_L / inst_chiN / inst_fA (length p.batch, each may
@@ -1051,7 +1050,7 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     }
     // reduction blocks per instance: ~2 points per thread, at most 2048 partials
     {
-        if (p.xy_plane < -1 || p.xy_plane > 2) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..2");
+        if (p.xy_plane < -1 || p.xy_plane > 3) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..3");
         if (p.xy_plane > 0 && !plane_fits(c->fp32, p.nx, p.ny, p.xy_plane))
             return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane needs nx == ny with a plane kernel that fits LDS "
                                                         "(16, 32, 48, 64; 128 except the FP64 xy-plane kernel)");
@@ -1059,7 +1058,10 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     }
     if (p.tiling < -1 || p.tiling >= PASS_VARIANTS)
         return set_err(nullptr, SCFT_E_INVALID, "tiling must be -1 or 0.." + std::to_string(PASS_VARIANTS - 1));
-    for (int a = 0; a < 3; ++a) c->pass_variant[a] = p.tiling < 0 ? 0 : p.tiling;
+    for (int a = 0; a < 3; ++a) {  // the three-level tiling only where the length has one
+        c->pass_variant[a] = p.tiling < 0 ? 0 : p.tiling;
+        if (c->pass_variant[a] == 2 && !pass3_supported(n[a])) c->pass_variant[a] = 0;
+    }
     c->nblk = (int)std::min<long>(2048, std::max<long>(1, (c->N + RT * 2 - 1) / (RT * 2)));
 
     int ndev = 0;
@@ -1284,10 +1286,9 @@ static hipError_t launch_unit(scft_ctx *c, int unit, const PassArgs<T> &pa, bool
         const long np = (long)c->batch * c->n[2];
         const C *twy = (const C *)c->tw[1];
         if (fused) return launch_plane_fused<T>(c->use_plane, c->n[0], pa, twy, kfy, np, c->stream);
-        return c->use_plane == 2 ? launch_split<T>(c->n[0], pa, twy, kfy, np, c->stream)
-                                 : launch_plane<T>(c->n[0], pa, twy, kfy, np, c->stream);
+        return launch_plane_form<T>(c->use_plane, c->n[0], pa, twy, kfy, np, c->stream);
     }
-    if (fused) return launch_fused<T>(unit, c->n[unit], pa, c->stream);
+    if (fused) return launch_fused<T>(unit, c->n[unit], c->pass_variant[unit], pa, c->stream);
     return launch_pass<T>(unit, c->n[unit], c->pass_variant[unit], pa, c->stream);
 }
 
@@ -1738,15 +1739,18 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.dst = (C *)c->work;
     pa.W = (const T *)c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
-    // step-boundary kernels (axis 5..9) store z(t) to a scratch slice
-    pa.slot = axis >= 5 ? slot_ptr<T>(c, 1) : nullptr;
+    // codes: 0..2 axis pass, 3/4/10 plane forms 1/2/3; 5..7 boundary axis
+    // kernels, 8/9/11 boundary plane forms 1/2/3 (these store z(t) to a scratch slice)
+    const bool fused = (axis >= 5 && axis <= 9) || axis == 11;
+    const int form = (axis == 3 || axis == 8) ? 1 : (axis == 4 || axis == 9) ? 2 : (axis == 10 || axis == 11) ? 3 : 0;
+    pa.slot = fused ? slot_ptr<T>(c, 1) : nullptr;
     pa.wn[0] = pa.wn[1] = 0u;
     pa.post_scale = (T)(1.0 / (double)c->N);
     pa.post_from_pre = c->post_exact;
     pa.nsab = (const int2 *)c->nsab;
     pa.step = 0;
     pa.BN = (unsigned)(c->batch * c->N);
-    const int ax = (axis >= 5 && axis <= 7) ? axis - 5 : (axis >= 3 ? 0 : axis);
+    const int ax = (axis >= 5 && axis <= 7) ? axis - 5 : (form ? 0 : axis);
     pa.kf_stride = c->kf_per_inst ? (unsigned)c->n[ax] : 0u;
     pa.tw = (const C *)c->tw[ax];
     pa.kf = (const typename kfac<T>::type *)c->kf[ax];
@@ -1761,11 +1765,10 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     const long np = (long)c->batch * c->n[2];
     for (int r = 0; r < reps; ++r) {
         const C *twy = (const C *)c->tw[1];
-        hipError_t e = axis >= 8   ? launch_plane_fused<T>(axis - 7, c->n[0], pa, twy, kfy, np, c->stream)
-                       : axis >= 5 ? launch_fused<T>(ax, c->n[ax], pa, c->stream)
-                       : axis == 4 ? launch_split<T>(c->n[0], pa, twy, kfy, np, c->stream)
-                       : axis == 3 ? launch_plane<T>(c->n[0], pa, twy, kfy, np, c->stream)
-                                   : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
+        hipError_t e = form && fused ? launch_plane_fused<T>(form, c->n[0], pa, twy, kfy, np, c->stream)
+                       : form        ? launch_plane_form<T>(form, c->n[0], pa, twy, kfy, np, c->stream)
+                       : fused       ? launch_fused<T>(ax, c->n[ax], variant, pa, c->stream)
+                                     : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1843,14 +1846,14 @@ static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, flo
 
 extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
-    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 9) return SCFT_E_INVALID;
-    const int form = axis == 3 || axis == 8 ? 1 : (axis == 4 || axis == 9 ? 2 : 0);
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 11) return SCFT_E_INVALID;
+    const int form = (axis == 3 || axis == 8) ? 1 : (axis == 4 || axis == 9) ? 2 : (axis == 10 || axis == 11) ? 3 : 0;
     if (form && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form))
         return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel of this form for this grid");
     const int line_axis = axis < 3 ? axis : (axis >= 5 && axis <= 7 ? axis - 5 : -1);
     if (line_axis >= 0 && !ctx->active[line_axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
     HIPX(ctx, hipSetDevice(ctx->device));
-    return time_pass_variant(ctx, axis, reps, axis < 3 ? ctx->pass_variant[axis] : 0, ms);
+    return time_pass_variant(ctx, axis, reps, line_axis >= 0 ? ctx->pass_variant[line_axis] : 0, ms);
 }
 
 // Autotuner: time every tiling variant of every active axis pass (and the
@@ -1891,7 +1894,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     int plane = ctx->p.xy_plane > 0 ? ctx->p.xy_plane : 0;
     if (ctx->p.xy_plane < 0) {
         float best = 0.0f;
-        for (int form = 0; form <= 2; ++form) {
+        for (int form = 0; form <= 3; ++form) {
             if (form > 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form)) continue;
             ctx->use_plane = form;
             float t = 0.0f;
@@ -1905,7 +1908,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     }
     float tplane = 0.0f;
     if (plane) {
-        int rc = time_pass_variant(ctx, 2 + plane, reps, 0, &tplane);
+        int rc = time_pass_variant(ctx, plane == 3 ? 10 : 2 + plane, reps, 0, &tplane);
         if (rc) return rc;
     }
     ctx->use_plane = plane;
@@ -1935,7 +1938,7 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
     for (int a = 0; a < 3; ++a)
         if (choice[a] < 0 || choice[a] >= PASS_VARIANTS)
             return set_err(ctx, SCFT_E_INVALID, "tiling variant must be 0.." + std::to_string(PASS_VARIANTS - 1));
-    if (choice[3] < 0 || choice[3] > 2) return set_err(ctx, SCFT_E_INVALID, "plane form must be 0, 1 or 2");
+    if (choice[3] < 0 || choice[3] > 3) return set_err(ctx, SCFT_E_INVALID, "plane form must be 0..3");
     if (choice[3] != 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], choice[3]))
         return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
     for (int a = 0; a < 3; ++a) {  // one trial launch: a variant whose LDS does not fit fails here
