@@ -103,6 +103,55 @@ CODES = {"x": 0, "y": 1, "z": 2, "xy_plane": 3, "xy_split": 4, "fused_x": 5, "fu
          "fused_plane": 8, "fused_split": 9, "fused_split3": 11}
 
 
+def c5_adjoint(pkg, n_dirs=2):
+    """c5: loss = ||phiA - phi_target||^2 after 50 unrolled Anderson iterations
+    (64^3, f = 0.5, chiN = 18, ds = 0.01, sqrt(Ns) checkpointing), gradient with
+    respect to an external field h on A by the discrete adjoint.  Reports the
+    time per gradient in both forms, central differences (eps = 1e-5) along
+    random directions of the FP64 gradient, and the FP32 gradient's distance
+    from the FP64 one."""
+    w = pkg.fields.WORKLOADS["c5_adjoint_64"]
+    n, L = w["n"], w["L"]
+    N = n ** 3
+    w0A, w0B = pkg.fields.workload_fields("c5_adjoint_64")
+    x = pkg.fields.coords(n, n, n, L, L, L)[0]
+    u = pkg.fields.splitmix_uniform(55, 2)
+    target = 0.5 + (0.2 + 0.2 * u[0]) * np.cos(2 * np.pi * x / L + 2 * np.pi * u[1])
+    h = np.zeros(N)
+    out = {}
+    grads = {}
+    for prec in (pkg.FP64, pkg.FP32):
+        p = pkg.default_params(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, chiN=w["chiN"], fA=w["fA"], ds=w["ds"],
+                               precision=prec, mixing=pkg.MIX_ANDERSON, anderson_m=10, anderson_mix=0.1,
+                               checkpoint=1)
+        with pkg.Solver(p) as s:
+            s.autotune(5)
+            s.adjoint_loop(h, w0A, w0B, 2, target)  # warm-up
+            t0 = time.perf_counter()
+            loss, g, _ = s.adjoint_loop(h, w0A, w0B, w["iters"], target)
+            dt = time.perf_counter() - t0
+            key = "fp32" if prec else "fp64"
+            out[f"{key}_s_per_gradient"] = dt
+            out[f"{key}_loss"] = float(loss[0])
+            grads[key] = g[0].copy()
+            if not prec:
+                rng = np.random.default_rng(5)
+                worst = 0.0
+                for _ in range(n_dirs):
+                    v = rng.standard_normal(N)
+                    eps = 1e-5
+                    lp = s.adjoint_loop(h + eps * v, w0A, w0B, w["iters"], target, grad=False)[0][0]
+                    lm = s.adjoint_loop(h - eps * v, w0A, w0B, w["iters"], target, grad=False)[0][0]
+                    fd = (lp - lm) / (2 * eps)
+                    ad = float(g[0] @ v)
+                    worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
+                out["fp64_fd_rel_mismatch"] = worst
+                out["fd_directions"] = n_dirs
+    out["fp32_vs_fp64_grad_rel"] = float(np.max(np.abs(grads["fp32"] - grads["fp64"])) / np.max(np.abs(grads["fp64"])))
+    out["iterations"] = w["iters"]
+    return out
+
+
 def pass_roofline(s, prec, reps=50):
     """Per-launch duration (HIP events around `reps` back-to-back launches on
     the solver stream) of every kernel the fused march of this solver's tiling
@@ -188,6 +237,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-128", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c3_fp32", "c3_fp64"],
                     help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
     ap.add_argument("--choice", default=None,
@@ -280,6 +330,10 @@ def main():
         t64.close()
         log("[bench] c3 one evaluation of both forms on the FP64 run's final fields")
         forms["c3_128"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0])
+        log("[bench] c3 FP32 with sqrt(Ns) checkpointing")
+        tc, mc, _, _ = run_form(pkg, name3, pkg.FP32, 4, 1, checkpoint=1)
+        forms["c3_128"]["fp32_ckpt_iters_per_s"] = 1e3 / mc
+        tc.close()
     if not a.no_sweep:
         log("[bench] c4 sweep: 256 instances, grids 32/48/64, FP64")
         inst = pkg.fields.sweep_instances(count=256, seed=4)
@@ -300,6 +354,9 @@ def main():
             * 1e3 / ms / 1e9 / HBM_PEAK_GBPS,
         }
         sw.close()
+    if not a.no_adjoint:
+        log("[bench] c5 adjoint: 64^3, 50 unrolled Anderson iterations, sqrt(Ns) checkpointing")
+        forms["c5_adjoint"] = c5_adjoint(pkg)
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, host cores)")

@@ -84,6 +84,10 @@ typedef struct {
     int fuse_steps;       /* 1: the packed march alternates its axis order and runs each
                              step boundary (the last axis of step t, the first of t+1) as
                              one kernel; 0: every step's passes separately */
+    double anderson_reg;  /* relative ridge of the Anderson least squares: U + anderson_reg *
+                             tr(U)/m * I (0, the default: plain least squares).  Bounds the
+                             condition number, so that derivatives through many unrolled
+                             Anderson iterations (scft_adjoint_loop) stay finite */
 } scft_params;
 
 /* Fill p with the reference's defaults (ab.txt sample sizes aside):

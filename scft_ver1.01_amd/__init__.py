@@ -78,6 +78,7 @@ class Params(ctypes.Structure):
         ("anderson_m", ctypes.c_int), ("anderson_mix", ctypes.c_double),
         ("pi", ctypes.c_double), ("use_graph", ctypes.c_int), ("xy_plane", ctypes.c_int),
         ("tiling", ctypes.c_int), ("checkpoint", ctypes.c_int), ("fuse_steps", ctypes.c_int),
+        ("anderson_reg", ctypes.c_double),
     ]
 
     def replace(self, **kw):

@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
 // one workgroup per instance: finish the new Gram row, solve, write coefficients
 __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ part, int nblk, int maxm, int m,
                                                   const long *cnt, double *__restrict__ G, double *__restrict__ coef,
-                                                  double *__restrict__ err)
+                                                  double *__restrict__ err, double ridge)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     __shared__ double row[AND_MAX + 2];
@@ -607,6 +607,12 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
             const int sj = rg.s[j + 1];
             U[i * mp + j] = gcc - g[c * m1 + si] - g[c * m1 + sj] + g[si * m1 + sj];
         }
+    }
+    if (ridge > 0.0 && mp > 0) {  // U += ridge * tr(U)/mp * I
+        double tr = 0.0;
+        for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
+        const double lam = ridge * tr / mp;
+        for (int i = 0; i < mp; ++i) U[i * mp + i] += lam;
     }
     // Gaussian elimination with partial pivoting (same steps as the oracle)
     bool ok = true;
@@ -813,6 +819,7 @@ extern "C" void scft_default_params(scft_params *p)
     p->use_graph = 1;
     p->xy_plane = -1;
     p->fuse_steps = 1;
+    p->anderson_reg = 0.0;
     p->tiling = -1;
     p->checkpoint = 0;
 }
@@ -1051,6 +1058,7 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     // reduction blocks per instance: ~2 points per thread, at most 2048 partials
     {
         if (p.xy_plane < -1 || p.xy_plane > 3) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..3");
+        if (!(p.anderson_reg >= 0.0)) return set_err(nullptr, SCFT_E_INVALID, "anderson_reg must be >= 0");
         if (p.xy_plane > 0 && !plane_fits(c->fp32, p.nx, p.ny, p.xy_plane))
             return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane needs nx == ny with a plane kernel that fits LDS "
                                                         "(16, 32, 48, 64; 128 except the FP64 xy-plane kernel)");
@@ -1519,7 +1527,7 @@ static hipError_t enqueue_update(scft_ctx *c)
                                c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
                                c->apart);
         hipLaunchKernelGGL(k_and_solve, dim3(c->batch), dim3(RT), 0, c->stream, c->apart, c->nblk, c->and_maxm, m,
-                           c->and_cnt, c->G, c->coef, c->aerr);
+                           c->and_cnt, c->G, c->coef, c->aerr, c->p.anderson_reg);
         hipLaunchKernelGGL(k_and_update, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch,
                            c->nblk, m, c->and_cnt, c->p.anderson_mix, (const double *)c->chiN_b, c->wh, c->dh,
                            c->coef, c->fpart);
@@ -1906,12 +1914,38 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
             }
         }
     }
+    ctx->use_plane = plane;
+    // refine the axis variants on whole marches: the boundary kernels follow
+    // their axis's variant, and a pass that wins alone need not win there
+    if (ctx->p.fuse_steps) {
+        float best = 0.0f;
+        int rc = time_march(ctx, 2, &best);
+        if (rc) return rc;
+        for (int a = plane ? 2 : 0; a < 3; ++a) {
+            if (!ctx->active[a]) continue;
+            const int keep = ctx->pass_variant[a];
+            for (int v = 0; v < PASS_VARIANTS; ++v) {
+                if (v == keep) continue;
+                float t = 0.0f;
+                if (time_pass_variant(ctx, a, 1, v, &t) != SCFT_OK) {  // no such tiling for this length
+                    (void)hipGetLastError();
+                    continue;
+                }
+                const int prev = ctx->pass_variant[a];
+                ctx->pass_variant[a] = v;
+                if ((rc = time_march(ctx, 2, &t))) return rc;
+                if (t < best) best = t;
+                else ctx->pass_variant[a] = prev;
+            }
+            per_axis[a] = 0.0f;
+            (void)time_pass_variant(ctx, a, reps, ctx->pass_variant[a], &per_axis[a]);
+        }
+    }
     float tplane = 0.0f;
     if (plane) {
         int rc = time_pass_variant(ctx, plane == 3 ? 10 : 2 + plane, reps, 0, &tplane);
         if (rc) return rc;
     }
-    ctx->use_plane = plane;
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr;
@@ -2127,6 +2161,7 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     const size_t BN = (size_t)nb * N;
     const double chiN = ctx->p.chiN, wcmp = ctx->p.wcmp;
     const bool anderson = ctx->p.mixing == SCFT_MIX_ANDERSON;
+    const double ridge = ctx->p.anderson_reg;
     const int m = anderson ? ctx->p.anderson_m : 0;
     const double lam = anderson ? ctx->p.anderson_mix : ctx->p.wopt;
     const int K = n_iter;
@@ -2198,6 +2233,11 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                         }
                         U[(i - 1) * mp + (j - 1)] = U[(j - 1) * mp + (i - 1)] = u;
                     }
+                }
+                if (ridge > 0.0) {  // U += ridge * tr(U)/mp * I, as k_and_solve
+                    double tr = 0.0;
+                    for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
+                    for (int i = 0; i < mp; ++i) U[i * mp + i] += ridge * tr / mp;
                 }
                 std::vector<double> x = V;
                 const bool ok = host_solve(mp, U, x);
@@ -2276,6 +2316,10 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                                           Uk[k].begin() + (size_t)(b + 1) * mp * mp);
                     std::vector<double> Vb = cbar;  // U symmetric: Vbar = U^-1 cbar
                     host_solve(mp, U, Vb);
+                    // the ridge ridge*tr(U)/mp*I adds -(ridge/mp)(Vbar.c) I to Ubar
+                    double rdiag = 0.0;
+                    if (ridge > 0.0)
+                        for (int i = 0; i < mp; ++i) rdiag -= ridge / mp * Vb[i] * c[i];
                     // Ubar_ij = -Vbar_i c_j;  Dbar_i = sum_j (Ubar_ij + Ubar_ji) Delta_j + Vbar_i d_k
                     for (int f = 0; f < 2; ++f) {
                         const double *d0 = inst(dd[k], f, b);
@@ -2289,6 +2333,7 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                                     const double Dj = d0[r] - inst(dd[k - j], f, b)[r];
                                     Dbi += (-Vb[i - 1] * c[j - 1] - Vb[j - 1] * c[i - 1]) * Dj;
                                 }
+                                Dbi += 2.0 * rdiag * Di;  // U_ii = <Delta_i, Delta_i>
                                 acc0 += Vb[i - 1] * Di + Dbi;
                                 inst(Db[k - i], f, b)[r] -= Dbi;
                             }
