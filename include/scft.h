@@ -180,7 +180,11 @@ int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const d
    each axis pass (`reps` launches each) and the fused xy-plane kernel, keeps
    the fastest.  choice (may be NULL): int[4] = variant of x, y, z pass (0..2),
    plane form used (0 none, 1 xy-plane, 2 split xy-plane, 3 split three-level);
-   ms (may be NULL): float[4] = best x, y, z pass and chosen plane-kernel times. */
+   ms (may be NULL): float[4] = best x, y, z pass and chosen plane-kernel times.
+   The plane form and the final axis variants are chosen by timing whole
+   marches on stand-in factors, which overwrites the W factors and the slice
+   store; fields, phi, Q and F are kept, and the next concentrations or
+   iteration recomputes the rest. */
 int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 
 /* Apply a tiling choice of the form scft_autotune returns (int[4]: variant of

@@ -37,6 +37,7 @@ struct Config {
     int pi_exact = 0;              // 1: use M_PI instead of the reference's 3.1415926535
     int autotune = 1;              // 1: time the kernel tilings and plane forms once before iterating
     int checkpoint = 0;            // 1: sqrt(Ns) checkpointing of the slice store
+    double anderson_reg = 0.0;     // relative ridge of the Anderson least squares
 };
 
 // parse flags; returns 0, or prints the problem and returns nonzero

@@ -118,6 +118,7 @@ int read_config(const std::string &path, Config &c, std::string &err)
         else if (key == "pi_exact") c.pi_exact = std::atoi(vs);
         else if (key == "autotune") c.autotune = std::atoi(vs);
         else if (key == "checkpoint") c.checkpoint = std::atoi(vs);
+        else if (key == "anderson_reg") c.anderson_reg = std::atof(vs);
         else {
             std::fclose(fp);
             err = path + ": unknown key '" + key + "'";

@@ -124,6 +124,7 @@ int main(int argc, char **argv)
     p.anderson_mix = c.anderson_mix;
     if (c.pi_exact) p.pi = 3.14159265358979323846;
     p.checkpoint = c.checkpoint;
+    p.anderson_reg = c.anderson_reg;
     scft_ctx *ctx = nullptr;
     int rc = scft_create(&p, c.which_gpus[0], &ctx);
     if (rc != SCFT_OK) {

@@ -40,7 +40,9 @@ def test_reference_ab_txt_parses(tmp_path):
 
 
 def test_gen_matches_python_generator(pkg, tmp_path):
-    write_ab(tmp_path / "ab.txt", extra="field=gyroid\namp=1.0\nnoise=0.1\nseed=9\n")
+    # the solver keys (autotune, checkpoint, anderson_reg) parse alongside the generator's
+    write_ab(tmp_path / "ab.txt", extra="field=gyroid\namp=1.0\nnoise=0.1\nseed=9\nautotune=0\ncheckpoint=1\n"
+                                      "anderson_reg=0.01\nmixing=anderson\n")
     r = run(["-gen"], str(tmp_path))
     assert r.returncode == 0, r.stdout
     for i in range(2):
