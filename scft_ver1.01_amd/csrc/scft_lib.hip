@@ -341,6 +341,46 @@ __global__ void __launch_bounds__(RT) k_contour_pair(const typename cplx<T>::typ
     phiB[i] = (init ? 0.0 : phiB[i]) + c * f;
 }
 
+// phi contributions of the pairs (tau, Ns - tau), tau = tau0..tau1 (checkpoint
+// mode, one launch per segment): z(tau0) at alo, z(tau0 + j) at abase + j*stride;
+// z(slo) at dlo, z(slo + j) at dbase + j*stride, slo = Ns - tau1.  The segment's
+// terms are summed in FP64 registers and added to phi once.
+template <typename T>
+__global__ void __launch_bounds__(RT) k_contour_seg(const typename cplx<T>::type *__restrict__ alo,
+                                                    const typename cplx<T>::type *__restrict__ abase,
+                                                    const typename cplx<T>::type *__restrict__ dlo,
+                                                    const typename cplx<T>::type *__restrict__ dbase, long stride,
+                                                    int tau0, int tau1, int NsA, int Ns, long N, int batch,
+                                                    const double *__restrict__ ffl, double *__restrict__ phiA,
+                                                    double *__restrict__ phiB, int init)
+{
+    const int b = blockIdx.y;
+    const long r = (long)blockIdx.x * RT + threadIdx.x;
+    if (r >= N) return;
+    const long i = b * N + r;
+    const int slo = Ns - tau1;
+    double a = 0.0, c = 0.0;
+    for (int tau = tau0; tau <= tau1; ++tau) {
+        const auto zt = tau == tau0 ? alo[i] : abase[(long)(tau - tau0) * stride + i];
+        const int u = Ns - tau;
+        const auto zu = u == slo ? dlo[i] : dbase[(long)(u - slo) * stride + i];
+        double wa, wb;
+        const double p1 = (double)zt.x * (double)zu.y;
+        trap_w(tau, NsA, Ns, wa, wb);
+        a += wa * p1;
+        c += wb * p1;
+        if (u != tau) {
+            const double p2 = (double)zu.x * (double)zt.y;
+            trap_w(u, NsA, Ns, wa, wb);
+            a += wa * p2;
+            c += wb * p2;
+        }
+    }
+    const double f = ffl[b];
+    phiA[i] = (init ? 0.0 : phiA[i]) + a * f;
+    phiB[i] = (init ? 0.0 : phiB[i]) + c * f;
+}
+
 // partial sums of lamA*phiA + lamB*phiB -> part[(b*2)*nblk + blk]
 __global__ void __launch_bounds__(RT) k_dot_lamphi(const double *__restrict__ lamA, const double *__restrict__ lamB,
                                                    const double *__restrict__ phiA, const double *__restrict__ phiB,
@@ -1231,7 +1271,10 @@ static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
     const int nck = Ns / K + 1;
     if (K_out) *K_out = K;
     if (nck_out) *nck_out = nck;
-    return ckpt ? (long)nck + K + 4 : (long)Ns + 3;
+    // checkpoint mode: nck checkpoints, a K-slice segment buffer (the pair
+    // walk's, and the segmented contour's ascending one), two ping-pong slices,
+    // the segmented contour's descending K-slice buffer, spare
+    return ckpt ? (long)nck + 2 * K + 4 : (long)Ns + 3;
 }
 
 // block of the real (br) and imaginary (bi) part at packed step t (1..Ns):
@@ -1358,24 +1401,26 @@ static typename cplx<T>::type *march_slot(scft_ctx *c, int t)
 // even steps in reverse: the last unit of step t is the first of step t+1, and
 // one kernel closes t (z(t) to its slot) and opens t+1 on the data in
 // registers.  Steps then cost nu - 1 launches instead of nu (nu >= 2 units).
-template <typename T>
-static hipError_t enqueue_march_fused(scft_ctx *c, int nsteps, typename cplx<T>::type **last)
+// z(t0) = src marched to z(t0 + nsteps); z(t) goes to dst_of(t)
+template <typename T, typename DstOf>
+static hipError_t enqueue_march_fused_from(scft_ctx *c, const typename cplx<T>::type *src0, int t0, int nsteps,
+                                           DstOf dst_of, typename cplx<T>::type **last)
 {
     using C = typename cplx<T>::type;
     C *work = (C *)c->work;
-    const C *slot0 = slot_ptr<T>(c, 0);
     int units[3];
     const int nu = step_units(c, units);
     C *dst = nullptr;
-    for (int t = 1; t <= nsteps; ++t) {
-        dst = march_slot<T>(c, t);
+    const int t_end = t0 + nsteps;
+    for (int t = t0 + 1; t <= t_end; ++t) {
+        dst = dst_of(t);
         for (int k = 0; k < nu; ++k) {
-            const int u = (t & 1) ? units[k] : units[nu - 1 - k];
+            const int u = ((t - t0) & 1) ? units[k] : units[nu - 1 - k];
             const bool first = k == 0, lastu = k == nu - 1;
-            if (first && t > 1) continue;  // opened by step t-1's boundary kernel
-            const C *src = (first && t == 1) ? slot0 : work;
+            if (first && t > t0 + 1) continue;  // opened by step t-1's boundary kernel
+            const C *src = (first && t == t0 + 1) ? src0 : work;
             hipError_t e;
-            if (lastu && t < nsteps) {
+            if (lastu && t < t_end) {
                 PassArgs<T> pa = pass_args<T>(c, u, src, work, t, 0, 0, first, false);
                 pa.slot = dst;
                 e = launch_unit<T>(c, u, pa, true);
@@ -1387,6 +1432,13 @@ static hipError_t enqueue_march_fused(scft_ctx *c, int nsteps, typename cplx<T>:
     }
     if (last) *last = dst;
     return hipSuccess;
+}
+
+template <typename T>
+static hipError_t enqueue_march_fused(scft_ctx *c, int nsteps, typename cplx<T>::type **last)
+{
+    return enqueue_march_fused_from<T>(c, slot_ptr<T>(c, 0), 0, nsteps, [c](int t) { return march_slot<T>(c, t); },
+                                       last);
 }
 
 // march `nsteps` steps from slot 0.  block -1: the packed chain; 0/1: one
@@ -1460,6 +1512,56 @@ struct PairWalk {
     }
 };
 
+// Checkpoint-mode contour by segments of K pairs: z(tau) for tau in
+// [tau0, tau1] (re-marched from checkpoint tau0 / K into buffer A) pairs with
+// z(Ns - tau) (re-marched from the checkpoint at or below Ns - tau1 into
+// buffer D, the slices before the range through the ping-pong pair), and one
+// k_contour_seg launch adds the segment's terms to phi.
+template <typename T>
+static hipError_t enqueue_contour_segments(scft_ctx *c, double *phiA, double *phiB)
+{
+    using C = typename cplx<T>::type;
+    const int Ns = c->Ns, K = c->ck_K, nck = c->ck_n;
+    // buffer A: z(tau0 + j) at slot a0 + j, j = 1..K-1 (the pair walk's segment
+    // buffer); buffer D: z(slo + j) at slot d0 + j, j = 0..K-1
+    const long a0 = nck, d0 = nck + K + 2;
+    const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+    auto march = [&](const C *src, int t0, int n, auto dst_of) -> hipError_t {
+        int units[3];
+        if (n <= 0) return hipSuccess;
+        if (c->p.fuse_steps && n > 1 && step_units(c, units) >= 2)
+            return enqueue_march_fused_from<T>(c, src, t0, n, dst_of, nullptr);
+        const C *cur = src;
+        for (int t = t0 + 1; t <= t0 + n; ++t) {
+            C *dst = dst_of(t);
+            hipError_t e = enqueue_step<T>(c, cur, dst, t, 0, 0);
+            if (e != hipSuccess) return e;
+            cur = dst;
+        }
+        return hipSuccess;
+    };
+    for (int tau0 = 0; 2 * tau0 <= Ns; tau0 += K) {
+        const int tau1 = std::min(tau0 + K - 1, Ns / 2);
+        // A: z(tau0 .. tau1); z(tau0) is checkpoint tau0 / K
+        const C *A0 = slot_ptr<T>(c, tau0 / K);
+        hipError_t e = march(A0, tau0, tau1 - tau0, [&](int t) { return slot_ptr<T>(c, a0 + (t - tau0)); });
+        if (e != hipSuccess) return e;
+        // D: z(s) for s in [Ns - tau1, Ns - tau0], marched from checkpoint k * K <= Ns - tau1
+        const int slo = Ns - tau1, shi = Ns - tau0, k = slo / K;
+        const C *D0 = slot_ptr<T>(c, k);
+        auto ddst = [&](int t) {
+            return t >= slo ? slot_ptr<T>(c, d0 + (t - slo)) : slot_ptr<T>(c, nck + K + (t & 1));
+        };
+        if ((e = march(D0, k * K, shi - k * K, ddst)) != hipSuccess) return e;
+        const C *Dlo = (slo == k * K) ? D0 : slot_ptr<T>(c, d0);
+        hipLaunchKernelGGL(k_contour_seg<T>, gc, dim3(RT), 0, c->stream, A0, (const C *)slot_ptr<T>(c, a0), Dlo,
+                           (const C *)slot_ptr<T>(c, d0), (long)c->batch * c->N, tau0, tau1, c->NsA, Ns, c->N,
+                           c->batch, (const double *)c->ffl, phiA, phiB, tau0 == 0 ? 1 : 0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // getConc on the device for fields (fA, fB); mean_part: partial sums of fA to
 // subtract first (or null: fields used as given)
 template <typename T>
@@ -1485,14 +1587,7 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
                            (const int2 *)c->nsab, c->ffl, phiA, phiB);
         return hipGetLastError();
     }
-    PairWalk<T> walk(c);
-    for (int tau = 0; 2 * tau <= c->Ns; ++tau) {
-        const C *zt, *zu;
-        if ((e = walk.next(tau, &zt, &zu)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_contour_pair<T>, gc, dim3(RT), 0, c->stream, zt, zu, tau, c->NsA, c->Ns, c->N,
-                           c->batch, (const double *)c->ffl, phiA, phiB, tau == 0 ? 1 : 0);
-    }
-    return hipGetLastError();
+    return enqueue_contour_segments<T>(c, phiA, phiB);
 }
 
 template <typename T>
@@ -2216,24 +2311,22 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
             std::vector<double> c(mp, 0.0);
             if (mp > 0) {
                 // U_ij = <d_k - d_{k-i}, d_k - d_{k-j}>, V_i = <d_k - d_{k-i}, d_k>
-                std::vector<double> U(mp * mp), V(mp);
-                for (int i = 1; i <= mp; ++i) {
-                    double v = 0.0;
-                    for (int f = 0; f < 2; ++f) {
-                        const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b);
-                        for (long r = 0; r < N; ++r) v += (d0[r] - di[r]) * d0[r];
-                    }
-                    V[i - 1] = v;
-                    for (int j = i; j <= mp; ++j) {
-                        double u = 0.0;
-                        for (int f = 0; f < 2; ++f) {
-                            const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b),
-                                         *dj = inst(dd[k - j], f, b);
-                            for (long r = 0; r < N; ++r) u += (d0[r] - di[r]) * (d0[r] - dj[r]);
+                // one pass over the points accumulates every <Delta_i, Delta_j> and <Delta_i, d_k>
+                std::vector<double> U(mp * mp, 0.0), V(mp, 0.0), Dl(mp);
+                for (int f = 0; f < 2; ++f) {
+                    const double *d0 = inst(dd[k], f, b);
+                    std::vector<const double *> dk(mp + 1);
+                    for (int i = 1; i <= mp; ++i) dk[i] = inst(dd[k - i], f, b);
+                    for (long r = 0; r < N; ++r) {
+                        for (int i = 0; i < mp; ++i) Dl[i] = d0[r] - dk[i + 1][r];
+                        for (int i = 0; i < mp; ++i) {
+                            V[i] += Dl[i] * d0[r];
+                            for (int j = i; j < mp; ++j) U[i * mp + j] += Dl[i] * Dl[j];
                         }
-                        U[(i - 1) * mp + (j - 1)] = U[(j - 1) * mp + (i - 1)] = u;
                     }
                 }
+                for (int i = 0; i < mp; ++i)
+                    for (int j = 0; j < i; ++j) U[i * mp + j] = U[j * mp + i];
                 if (ridge > 0.0) {  // U += ridge * tr(U)/mp * I, as k_and_solve
                     double tr = 0.0;
                     for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
@@ -2321,21 +2414,30 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                     if (ridge > 0.0)
                         for (int i = 0; i < mp; ++i) rdiag -= ridge / mp * Vb[i] * c[i];
                     // Ubar_ij = -Vbar_i c_j;  Dbar_i = sum_j (Ubar_ij + Ubar_ji) Delta_j + Vbar_i d_k
+                    // Dbar_i = Vbar_i (d_k - S1) - c_i S2 + 2 rdiag Delta_i with
+                    // S1 = sum_j c_j Delta_j, S2 = sum_j Vbar_j Delta_j (O(m) per point)
                     for (int f = 0; f < 2; ++f) {
                         const double *d0 = inst(dd[k], f, b);
                         double *db0 = inst(Db[k], f, b);
+                        std::vector<const double *> dk(mp + 1);
+                        std::vector<double *> dbk(mp + 1);
+                        for (int i = 1; i <= mp; ++i) {
+                            dk[i] = inst(dd[k - i], f, b);
+                            dbk[i] = inst(Db[k - i], f, b);
+                        }
                         for (long r = 0; r < N; ++r) {
+                            double S1 = 0.0, S2 = 0.0;
+                            for (int j = 1; j <= mp; ++j) {
+                                const double Dj = d0[r] - dk[j][r];
+                                S1 += c[j - 1] * Dj;
+                                S2 += Vb[j - 1] * Dj;
+                            }
                             double acc0 = 0.0;
                             for (int i = 1; i <= mp; ++i) {
-                                const double Di = d0[r] - inst(dd[k - i], f, b)[r];
-                                double Dbi = Vb[i - 1] * d0[r];
-                                for (int j = 1; j <= mp; ++j) {
-                                    const double Dj = d0[r] - inst(dd[k - j], f, b)[r];
-                                    Dbi += (-Vb[i - 1] * c[j - 1] - Vb[j - 1] * c[i - 1]) * Dj;
-                                }
-                                Dbi += 2.0 * rdiag * Di;  // U_ii = <Delta_i, Delta_i>
+                                const double Di = d0[r] - dk[i][r];
+                                const double Dbi = Vb[i - 1] * (d0[r] - S1) - c[i - 1] * S2 + 2.0 * rdiag * Di;
                                 acc0 += Vb[i - 1] * Di + Dbi;
-                                inst(Db[k - i], f, b)[r] -= Dbi;
+                                dbk[i][r] -= Dbi;
                             }
                             db0[r] += acc0;
                         }
