@@ -312,35 +312,6 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
     phiB[b * N + r] = accB * f;
 }
 
-// phi contributions of the pair (s, Ns - s) from z(s) = zs and z(Ns - s) = zu
-// (checkpoint mode: one launch per pair, accumulating into phi; ffl applied
-// per term so phi is complete after the last pair)
-template <typename T>
-__global__ void __launch_bounds__(RT) k_contour_pair(const typename cplx<T>::type *__restrict__ zs,
-                                                     const typename cplx<T>::type *__restrict__ zu, int s, int NsA,
-                                                     int Ns, long N, int batch, const double *__restrict__ ffl,
-                                                     double *__restrict__ phiA, double *__restrict__ phiB, int init)
-{
-    const int b = blockIdx.y;
-    const long r = (long)blockIdx.x * RT + threadIdx.x;
-    if (r >= N) return;
-    const long i = b * N + r;
-    const double f = ffl[b];
-    double wa, wb;
-    const int u = Ns - s;
-    const double p1 = (double)zs[i].x * (double)zu[i].y;
-    trap_w(s, NsA, Ns, wa, wb);
-    double a = wa * p1, c = wb * p1;
-    if (u != s) {
-        const double p2 = (double)zu[i].x * (double)zs[i].y;
-        trap_w(u, NsA, Ns, wa, wb);
-        a += wa * p2;
-        c += wb * p2;
-    }
-    phiA[i] = (init ? 0.0 : phiA[i]) + a * f;
-    phiB[i] = (init ? 0.0 : phiB[i]) + c * f;
-}
-
 // phi contributions of the pairs (tau, Ns - tau), tau = tau0..tau1 (checkpoint
 // mode, one launch per segment): z(tau0) at alo, z(tau0 + j) at abase + j*stride;
 // z(slo) at dlo, z(slo + j) at dbase + j*stride, slo = Ns - tau1.  The segment's
