@@ -250,66 +250,84 @@ __device__ __forceinline__ void trap_w(int s, int NsA, int Ns, double &wa, doubl
     wb = (s > NsA) ? ((s == Ns) ? 0.5 : 1.0) : ((s == NsA) ? 0.5 : 0.0);
 }
 
+// P consecutive complex elements of a slice, loaded as one 16-byte access
+template <typename T, int P> struct alignas(P * sizeof(typename cplx<T>::type)) ZGroup {
+    typename cplx<T>::type z[P];
+};
+
 // phi by the trapezoid rule with FP64 accumulation (w_to_phi_go, cuda_aid.cu:61-118).
 // Contour point s pairs q(s) = Re z(s) with qdag(s) = Im z(Ns - s); slices s and
-// Ns - s are loaded once for both products.  One point per thread, 4 slice
-// pairs in flight.
-template <typename T>
+// Ns - s are loaded once for both products.  P consecutive points per thread
+// (one 16-byte load per slice: P = 2 in FP32, 1 in FP64; N % P == 0), 4 slice
+// pairs in flight.  Each point's sum runs in the order of the reference's
+// s-loop restricted to the pairs, the same for every P.
+template <typename T, int P>
 __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__restrict__ slots, long N, int batch,
                                                 const int2 *__restrict__ nsab, const double *__restrict__ ffl,
                                                 double *__restrict__ phiA, double *__restrict__ phiB)
 {
-    using C = typename cplx<T>::type;
+    using G = ZGroup<T, P>;
     const int b = blockIdx.y;
     const int NsA = nsab[b].x, Ns = nsab[b].x + nsab[b].y;
-    const long r = (long)blockIdx.x * RT + threadIdx.x;
-    if (r >= N) return;
-    const long BN = (long)batch * N;
-    const C *zp = slots + b * N + r;
-    double accA = 0.0, accB = 0.0;
+    const long g = (long)blockIdx.x * RT + threadIdx.x;
+    if (g * P >= N) return;
+    const long BNg = (long)batch * N / P;
+    const G *zp = reinterpret_cast<const G *>(slots + b * N) + g;
+    double accA[P], accB[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) accA[p] = accB[p] = 0.0;
     const int half = Ns / 2;  // pairs s = 0..half (s == Ns - s only when Ns is even and s == half)
     int s = 0;
     for (; s + 3 <= half; s += 4) {
-        C zs[4], zu[4];
+        G zs[4], zu[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            zs[k] = zp[(long)(s + k) * BN];
-            zu[k] = zp[(long)(Ns - s - k) * BN];
+            zs[k] = zp[(long)(s + k) * BNg];
+            zu[k] = zp[(long)(Ns - s - k) * BNg];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int s1 = s + k, u1 = Ns - s - k;
-            double wa, wb;
-            const double p1 = (double)zs[k].x * (double)zu[k].y;  // q(s1) qdag(s1)
-            trap_w(s1, NsA, Ns, wa, wb);
-            accA += wa * p1;
-            accB += wb * p1;
-            if (u1 != s1) {
-                const double p2 = (double)zu[k].x * (double)zs[k].y;  // q(u1) qdag(u1)
-                trap_w(u1, NsA, Ns, wa, wb);
-                accA += wa * p2;
-                accB += wb * p2;
+            double wa1, wb1, wa2, wb2;
+            trap_w(s1, NsA, Ns, wa1, wb1);
+            trap_w(u1, NsA, Ns, wa2, wb2);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const double p1 = (double)zs[k].z[p].x * (double)zu[k].z[p].y;  // q(s1) qdag(s1)
+                accA[p] += wa1 * p1;
+                accB[p] += wb1 * p1;
+                if (u1 != s1) {
+                    const double p2 = (double)zu[k].z[p].x * (double)zs[k].z[p].y;  // q(u1) qdag(u1)
+                    accA[p] += wa2 * p2;
+                    accB[p] += wb2 * p2;
+                }
             }
         }
     }
     for (; s <= half; ++s) {
         const int u1 = Ns - s;
-        const C a = zp[(long)s * BN], c = zp[(long)u1 * BN];
-        double wa, wb;
-        const double p1 = (double)a.x * (double)c.y;
-        trap_w(s, NsA, Ns, wa, wb);
-        accA += wa * p1;
-        accB += wb * p1;
-        if (u1 != s) {
-            const double p2 = (double)c.x * (double)a.y;
-            trap_w(u1, NsA, Ns, wa, wb);
-            accA += wa * p2;
-            accB += wb * p2;
+        const G a = zp[(long)s * BNg], c = zp[(long)u1 * BNg];
+        double wa1, wb1, wa2, wb2;
+        trap_w(s, NsA, Ns, wa1, wb1);
+        trap_w(u1, NsA, Ns, wa2, wb2);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double p1 = (double)a.z[p].x * (double)c.z[p].y;
+            accA[p] += wa1 * p1;
+            accB[p] += wb1 * p1;
+            if (u1 != s) {
+                const double p2 = (double)c.z[p].x * (double)a.z[p].y;
+                accA[p] += wa2 * p2;
+                accB[p] += wb2 * p2;
+            }
         }
     }
     const double f = ffl[b];
-    phiA[b * N + r] = accA * f;
-    phiB[b * N + r] = accB * f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        phiA[b * N + g * P + p] = accA[p] * f;
+        phiB[b * N + g * P + p] = accB[p] * f;
+    }
 }
 
 // phi contributions of the pairs (tau, Ns - tau), tau = tau0..tau1 (checkpoint
@@ -1552,10 +1570,16 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
                        c->ckpt ? (const int2 *)nullptr : (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
                        c->ffl);
-    const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     if (!c->ckpt) {
-        hipLaunchKernelGGL(k_contour<T>, gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+        if (sizeof(C) == 8 && c->N % 2 == 0) {
+            const dim3 gc((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
+            hipLaunchKernelGGL((k_contour<T, 2>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                               (const int2 *)c->nsab, c->ffl, phiA, phiB);
+        } else {
+            const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+            hipLaunchKernelGGL((k_contour<T, 1>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                               (const int2 *)c->nsab, c->ffl, phiA, phiB);
+        }
         return hipGetLastError();
     }
     return enqueue_contour_segments<T>(c, phiA, phiB);
