@@ -38,12 +38,43 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Ridge on the Anderson least squares (U += ridge * tr(U)/m * I) for both forms
+# of the c2/c3 runs.  Without it the FP32 form's history of nearly collinear
+# residual differences amplifies FP32 rounding, and at 128^3 with m = 20 the
+# iteration leaves the basin for tens of iterations before it returns.
+ANDERSON_RIDGE = 1e-2
+
+
 def workload_params(pkg, name, prec, **kw):
     w = pkg.fields.WORKLOADS[name]
     n = w["n"]
+    kw.setdefault("anderson_reg", ANDERSON_RIDGE)
     return pkg.default_params(nx=n, ny=n, nz=n, lx=w["L"], ly=w["L"], lz=w["L"], chiN=w["chiN"],
                               fA=w["fA"], ds=w["ds"], precision=prec, mixing=pkg.MIX_ANDERSON,
                               anderson_m=w["anderson_m"], anderson_mix=0.1, **kw)
+
+
+def convergence(pkg, name, n_iter, choices):
+    """Both forms from the same seeded fields for n_iter iterations (ridge
+    ANDERSON_RIDGE): F and the incompressibility error max|1 - phiA - phiB|
+    at iterations 10, 25, 50, 100, ... and |dF| between the forms."""
+    wA, wB = pkg.fields.workload_fields(name)
+    marks = [k for k in (10, 25, 50, 100, 200, 400) if k <= n_iter]
+    out = {}
+    for prec, ch in ((pkg.FP64, choices[0]), (pkg.FP32, choices[1])):
+        with pkg.Solver(workload_params(pkg, name, prec)) as s:
+            s.set_tiling(ch)
+            s.set_fields(wA, wB)
+            done, hist = 0, {}
+            for k in marks:
+                _, fe = s.iterate(k - done)
+                done = k
+                hist[k] = (float(fe[0][0]), float(fe[0][4]))
+            out[prec] = hist
+    return {"iterations": marks, "ridge": ANDERSON_RIDGE,
+            "F_fp64": [out[pkg.FP64][k][0] for k in marks], "err_fp64": [out[pkg.FP64][k][1] for k in marks],
+            "F_fp32": [out[pkg.FP32][k][0] for k in marks], "err_fp32": [out[pkg.FP32][k][1] for k in marks],
+            "dF": [abs(out[pkg.FP32][k][0] - out[pkg.FP64][k][0]) for k in marks]}
 
 
 def iter_bytes(N, Ns, b):
@@ -333,6 +364,8 @@ def main():
         t64.close()
         log("[bench] c3 one evaluation of both forms on the FP64 run's final fields")
         forms["c3_128"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0])
+        log("[bench] c3 100 iterations of both forms from the seeded fields")
+        forms["c3_128"]["convergence"] = convergence(pkg, name3, 100, (t64.tuning[0], t32.tuning[0]))
         log("[bench] c3 FP32 with sqrt(Ns) checkpointing")
         tc, mc, _, _ = run_form(pkg, name3, pkg.FP32, 4, 1, checkpoint=1)
         forms["c3_128"]["fp32_ckpt_iters_per_s"] = 1e3 / mc
