@@ -7,9 +7,11 @@ over Ns = 200 contour steps, Q, the trapezoid contour integral, the Anderson
 field update and the free energy) of the cylinder workload: AB diblock f=0.3,
 chiN=20, 64^3 periodic grid, box 4.2 Rg, ds=0.005, Anderson history 10, seeded
 hexagonal-star fields.  `value` is the FP32-propagator form (FP64 Q, phi, F,
-fields and least squares); the FP64 form, the 128^3 gyroid workload and the
-FP32-vs-FP64 error are reported beside it.  Fields are resident in HBM before
-the timed region; timing uses HIP events on the solver's stream.
+fields and least squares); the FP64 form, the FP32X form (FP32 storage and
+HBM bytes, FP64 transform arithmetic), the 128^3 gyroid workload and the
+errors of both FP32-storage forms against FP64 are reported beside it.
+Fields are resident in HBM before the timed region; timing uses HIP events
+on the solver's stream.
 """
 import argparse
 import importlib.util
@@ -104,12 +106,14 @@ def run_form(pkg, name, prec, steps, warmup, choice=None, **kw):
     return s, ms / steps, fe[0].copy(), phiA[0].copy()
 
 
-def single_eval_error(pkg, name, wA, wB):
-    """One evaluation (getConc, field update, F) of the FP32 and FP64 forms on
-    identical fields: |dQ|/Q, |dF| and max|dphi| of the FP32 form against the
-    FP64 form (which matches the FP64 oracle to ~1e-12 on every tested grid)."""
+def single_eval_error(pkg, name, wA, wB, prec32=None):
+    """One evaluation (getConc, field update, F) of an FP32-storage form
+    (FP32 by default, or FP32X) and the FP64 form on identical fields: |dQ|/Q,
+    |dF| and max|dphi| against the FP64 form (which matches the FP64 oracle to
+    ~1e-12 on every tested grid)."""
+    prec32 = pkg.FP32 if prec32 is None else prec32
     out = {}
-    for prec in (pkg.FP64, pkg.FP32):
+    for prec in (pkg.FP64, prec32):
         p = workload_params(pkg, name, prec)
         with pkg.Solver(p) as s:
             s.set_fields(wA, wB)
@@ -119,7 +123,7 @@ def single_eval_error(pkg, name, wA, wB):
             fe = s.free_energy()[0]
             out[prec] = (Q, fe[0], phiA[0].copy(), phiB[0].copy())
     q64, f64, a64, b64 = out[pkg.FP64]
-    q32, f32, a32, b32 = out[pkg.FP32]
+    q32, f32, a32, b32 = out[prec32]
     return {"relQ": abs(q32 - q64) / q64, "dF": abs(f32 - f64), "F_fp64": f64,
             "max_dphiA": float(np.max(np.abs(a32 - a64))), "max_dphiB": float(np.max(np.abs(b32 - b64)))}
 
@@ -272,7 +276,7 @@ def main():
     ap.add_argument("--no-128", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c3_fp32", "c3_fp64"],
+    ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c2_fp32x", "c3_fp32", "c3_fp64", "c3_fp32x"],
                     help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
     ap.add_argument("--choice", default=None,
                     help="with --profile: tiling 'x,y,z,plane' to apply instead of autotuning "
@@ -293,7 +297,7 @@ def main():
     if a.profile:
         wl, form = a.profile.split("_")
         nm = {"c2": "c2_cylinder_64", "c3": "c3_gyroid_128"}[wl]
-        prec = pkg.FP32 if form == "fp32" else pkg.FP64
+        prec = {"fp32": pkg.FP32, "fp32x": pkg.FP32X, "fp64": pkg.FP64}[form]
         choice = [int(c) for c in a.choice.split(",")] if a.choice else None
         if a.roofline_only:
             if choice is None:
@@ -320,9 +324,11 @@ def main():
     name = "c2_cylinder_64"
     s32, ms32, fe32, phi32 = run_form(pkg, name, pkg.FP32, a.steps, a.warmup)
     s64, ms64, fe64, phi64 = run_form(pkg, name, pkg.FP64, a.steps, a.warmup)
+    sx, msx, fex, phix = run_form(pkg, name, pkg.FP32X, a.steps, a.warmup)
     Ns = s32.NsA + s32.NsB
     rl32 = pass_roofline(s32, 1)
     rl64 = pass_roofline(s64, 0)
+    rlx = pass_roofline(sx, 2)
     dom = rl32["dominant"]
     N = s32.N
     it32 = 1e3 / ms32
@@ -336,18 +342,30 @@ def main():
             "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
             "F_fp64": fe64[0], "iterations_compared": a.warmup + a.steps,
             "kernels_fp32": rl32, "kernels_fp64": rl64,
+            "fp32x_iters_per_s": 1e3 / msx, "ratio_fp32x_over_fp64": ms64 / msx,
+            "fp32x_iter_hbm_frac": iter_bytes(N, Ns, 4) * (1e3 / msx) / 1e9 / HBM_PEAK_GBPS,
+            "trajectory_dF_fp32x_vs_fp64": abs(fex[0] - fe64[0]),
+            "trajectory_max_dphiA_fp32x_vs_fp64": float(np.max(np.abs(phix - phi64))),
+            "kernels_fp32x": rlx,
         }
     }
     _, _, wa64, wb64 = s64.get_fields()
     s32.close()
     s64.close()
+    sx.close()
     forms["c2_64"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name, wa64[0], wb64[0])
+    forms["c2_64"]["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name, wa64[0], wb64[0], pkg.FP32X)
+    forms["c2_64"]["single_eval_seeded_fields"] = {
+        "fp32": single_eval_error(pkg, name, *pkg.fields.workload_fields(name)),
+        "fp32x": single_eval_error(pkg, name, *pkg.fields.workload_fields(name), pkg.FP32X)}
     if not a.no_128:
         name3 = "c3_gyroid_128"
         t32, m32, f32, p32 = run_form(pkg, name3, pkg.FP32, max(5, a.steps // 2), 3)
         t64, m64, f64, p64 = run_form(pkg, name3, pkg.FP64, max(5, a.steps // 2), 3)
+        tx, mx, fx, px = run_form(pkg, name3, pkg.FP32X, max(5, a.steps // 2), 3)
         rl3_32 = pass_roofline(t32, 1, reps=20)
         rl3_64 = pass_roofline(t64, 0, reps=20)
+        rl3_x = pass_roofline(tx, 2, reps=20)
         _, _, wa64, wb64 = t64.get_fields()
         N3 = t32.N
         Ns3 = t32.NsA + t32.NsB
@@ -359,11 +377,21 @@ def main():
             "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
             "kernels_fp32": rl3_32, "kernels_fp64": rl3_64,
             "tiling_fp32": t32.tuning[0], "tiling_fp64": t64.tuning[0],
+            "fp32x_iters_per_s": 1e3 / mx, "ratio_fp32x_over_fp64": m64 / mx,
+            "fp32x_iter_hbm_frac": iter_bytes(N3, Ns3, 4) * (1e3 / mx) / 1e9 / HBM_PEAK_GBPS,
+            "trajectory_dF_fp32x_vs_fp64": abs(fx[0] - f64[0]),
+            "trajectory_max_dphiA_fp32x_vs_fp64": float(np.max(np.abs(px - p64))),
+            "kernels_fp32x": rl3_x, "tiling_fp32x": tx.tuning[0],
         }
         t32.close()
         t64.close()
-        log("[bench] c3 one evaluation of both forms on the FP64 run's final fields")
+        tx.close()
+        log("[bench] c3 one evaluation of the forms on the FP64 run's final fields and on the seeded fields")
         forms["c3_128"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0])
+        forms["c3_128"]["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0], pkg.FP32X)
+        forms["c3_128"]["single_eval_seeded_fields"] = {
+            "fp32": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3)),
+            "fp32x": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3), pkg.FP32X)}
         log("[bench] c3 100 iterations of both forms from the seeded fields")
         forms["c3_128"]["convergence"] = convergence(pkg, name3, 100, (t64.tuning[0], t32.tuning[0]))
         log("[bench] c3 FP32 with sqrt(Ns) checkpointing")
@@ -419,7 +447,7 @@ def main():
         "config": {"workload": "c2 cylinder: AB diblock f=0.3 chiN=20, 64^3, L=4.2 Rg, ds=0.005 "
                                "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
                    "batch": 1},
-        "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0]},
+        "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0], "c2_fp32x": sx.tuning[0]},
         "roofline": {"bound": "hbm", "kernel": rl32["kernel"] + " (fp32, 64^3)", "achieved": rl32["achieved"],
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": rl32["achieved"] / HBM_PEAK_GBPS,
                      "traffic": traffic, "bytes_per_launch": rl32["bytes_per_launch"][dom],

@@ -44,6 +44,9 @@ extern "C" {
 /* propagator precision */
 #define SCFT_FP64 0 /* everything FP64 (the reference's arithmetic) */
 #define SCFT_FP32 1 /* propagator, FFT and stored q in FP32; Q, phi, F, fields FP64 */
+#define SCFT_FP32X 2 /* stored q, W and every pass's loads and stores in FP32 (the HBM
+                        bytes of SCFT_FP32), the transform arithmetic in FP64 registers
+                        and LDS (rounded to FP32 once per store); Q, phi, F, fields FP64 */
 
 /* field update */
 #define SCFT_MIX_SIMPLE 0   /* phi_w of the reference (cuda_aid.cu:382-416) */
@@ -61,7 +64,7 @@ typedef struct {
     double fA;            /* fa of ab.txt */
     double ds;            /* ds0 of ab.txt */
     int batch;            /* independent instances sharing these parameters (batch of ab.txt) */
-    int precision;        /* SCFT_FP64 or SCFT_FP32 */
+    int precision;        /* SCFT_FP64, SCFT_FP32 or SCFT_FP32X */
     int mixing;           /* SCFT_MIX_SIMPLE or SCFT_MIX_ANDERSON */
     double wopt;          /* simple-mixing fraction; reference 0.05 (cuda_aid.cu:395) */
     double wcmp;          /* incompressibility penalty; reference 0.10 (cuda_aid.cu:396) */

@@ -39,6 +39,7 @@ SCFT_E_STATE = 7
 
 FP64 = 0
 FP32 = 1
+FP32X = 2  # FP32 storage and HBM bytes, FP64 transform arithmetic
 MIX_SIMPLE = 0
 MIX_ANDERSON = 1
 REFERENCE_PI = 3.1415926535  # struct.h:11

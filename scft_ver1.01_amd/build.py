@@ -34,7 +34,7 @@ DEVICE_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["pass_f64.hip", "pass_f32.hip", "scft_lib.hip"]
+LIB_SOURCES = ["pass_f64.hip", "pass_f32.hip", "pass_f32x.hip", "scft_lib.hip"]
 
 
 def _run(cmd):
@@ -59,7 +59,8 @@ def build_lib(jobs=4):
             if out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    # -z defs: an object missing from the link is an error, not a load-time failure
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

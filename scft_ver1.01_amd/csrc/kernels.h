@@ -93,3 +93,21 @@ hipError_t launch_plane_fused(int form, int n, const PassArgs<T> &a, const typen
 void pass_tiling(int axis, int n, int is_fp32, int *A, int *B, int *lines, int *threads);
 
 }  // namespace scft
+
+// The same launchers over FP32 storage with FP64 arithmetic (pass_f32x.hip):
+// PassArgs<float> pointers for the field, W and slices; tw and kf point to
+// FP64 tables (double2 / double entries) despite their declared type.
+namespace scft_f32x {
+template <typename T>
+hipError_t launch_pass(int axis, int n, int variant, const scft::PassArgs<T> &a, hipStream_t s);
+template <typename T>
+bool plane_form_supported(int form, int n);
+template <typename T>
+hipError_t launch_plane_form(int form, int n, const scft::PassArgs<T> &a, const typename scft::cplx<T>::type *twy,
+                             const typename scft::kfac<T>::type *kfy, long nplanes, hipStream_t s);
+template <typename T>
+hipError_t launch_fused(int axis, int n, int variant, const scft::PassArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_plane_fused(int form, int n, const scft::PassArgs<T> &a, const typename scft::cplx<T>::type *twy,
+                              const typename scft::kfac<T>::type *kfy, long nplanes, hipStream_t s);
+}  // namespace scft_f32x

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -751,7 +752,8 @@ struct scft_ctx {
     int NsA = 0, NsB = 0, Ns = 0;
     long N = 0;
     int batch = 1;
-    int fp32 = 0;
+    int fp32 = 0;   // FP32 storage of the propagator (SCFT_FP32 and SCFT_FP32X)
+    int mixed = 0;  // SCFT_FP32X: FP64 transform arithmetic over the FP32 storage
     int n[3] = {1, 1, 1};
     int active[3] = {0, 0, 0};
     int first_axis = -1, last_axis = -1;
@@ -947,7 +949,7 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
             twr[k] = (double)cosl(ang);
             twi[k] = (double)-sinl(ang);
         }
-        if (c->fp32) {
+        if (c->fp32 && !c->mixed) {
             std::vector<float> kff(nk), tw(2 * n);
             for (size_t k = 0; k < nk; ++k) kff[k] = (float)kfa[a][k];
             for (int k = 0; k < n; ++k) {
@@ -976,10 +978,37 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
 // true if xy-plane form `form` (1 xy-plane, 2 split xy-plane, 3 split with the
 // three-level tiling) has a kernel for
 // an nx x ny plane in this precision that fits a CU's LDS
-static bool plane_fits(bool fp32, int nx, int ny, int form)
+static bool plane_fits(const scft_ctx *c, int nx, int ny, int form)
 {
     if (nx != ny || nx < 2 || form < 1 || form > 3) return false;
-    return fp32 ? plane_form_supported<float>(form, nx) : plane_form_supported<double>(form, nx);
+    if (c->mixed) return scft_f32x::plane_form_supported<float>(form, nx);
+    return c->fp32 ? plane_form_supported<float>(form, nx) : plane_form_supported<double>(form, nx);
+}
+
+// The pass launchers of the context's form: T = float selects the FP32
+// arithmetic kernels or, for SCFT_FP32X, the FP64-arithmetic ones over the
+// same FP32 storage (whose twiddle and k tables are FP64).
+template <typename T>
+static hipError_t run_pass(const scft_ctx *c, int axis, int n, int variant, const PassArgs<T> &a, bool fused)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (c->mixed)
+            return fused ? scft_f32x::launch_fused<float>(axis, n, variant, a, c->stream)
+                         : scft_f32x::launch_pass<float>(axis, n, variant, a, c->stream);
+    }
+    return fused ? launch_fused<T>(axis, n, variant, a, c->stream) : launch_pass<T>(axis, n, variant, a, c->stream);
+}
+template <typename T>
+static hipError_t run_plane(const scft_ctx *c, int form, const PassArgs<T> &a, const typename cplx<T>::type *twy,
+                            const typename kfac<T>::type *kfy, long np, bool fused)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (c->mixed)
+            return fused ? scft_f32x::launch_plane_fused<float>(form, c->n[0], a, twy, kfy, np, c->stream)
+                         : scft_f32x::launch_plane_form<float>(form, c->n[0], a, twy, kfy, np, c->stream);
+    }
+    return fused ? launch_plane_fused<T>(form, c->n[0], a, twy, kfy, np, c->stream)
+                 : launch_plane_form<T>(form, c->n[0], a, twy, kfy, np, c->stream);
 }
 
 // Context constructor.  inst_L / inst_chiN / inst_fA (length p.batch, each may
@@ -1000,8 +1029,8 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     if (!(p.fA > 0.0 && p.fA < 1.0)) return set_err(nullptr, SCFT_E_INVALID, "fA must be in (0, 1)");
     if (!std::isfinite(p.chiN)) return set_err(nullptr, SCFT_E_INVALID, "chiN must be finite");
     if (p.batch < 1) return set_err(nullptr, SCFT_E_INVALID, "batch must be >= 1");
-    if (p.precision != SCFT_FP64 && p.precision != SCFT_FP32)
-        return set_err(nullptr, SCFT_E_INVALID, "precision must be SCFT_FP64 or SCFT_FP32");
+    if (p.precision != SCFT_FP64 && p.precision != SCFT_FP32 && p.precision != SCFT_FP32X)
+        return set_err(nullptr, SCFT_E_INVALID, "precision must be SCFT_FP64, SCFT_FP32 or SCFT_FP32X");
     if (p.mixing != SCFT_MIX_SIMPLE && p.mixing != SCFT_MIX_ANDERSON)
         return set_err(nullptr, SCFT_E_INVALID, "mixing must be SCFT_MIX_SIMPLE or SCFT_MIX_ANDERSON");
     if (p.mixing == SCFT_MIX_ANDERSON && (p.anderson_m < 0 || p.anderson_m > AND_MAX))
@@ -1068,7 +1097,8 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     c->blocks_uniform = blocks_uniform;
     c->N = (long)p.nx * p.ny * p.nz;
     c->batch = p.batch;
-    c->fp32 = p.precision == SCFT_FP32;
+    c->fp32 = p.precision != SCFT_FP64;
+    c->mixed = p.precision == SCFT_FP32X;
     c->csize = c->fp32 ? 8 : 16;
     c->tsize = c->fp32 ? 4 : 8;
     // the pass kernels address the four W arrays ([4][batch][N]) with 32-bit byte offsets
@@ -1088,7 +1118,7 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     {
         if (p.xy_plane < -1 || p.xy_plane > 3) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..3");
         if (!(p.anderson_reg >= 0.0)) return set_err(nullptr, SCFT_E_INVALID, "anderson_reg must be >= 0");
-        if (p.xy_plane > 0 && !plane_fits(c->fp32, p.nx, p.ny, p.xy_plane))
+        if (p.xy_plane > 0 && !plane_fits(c.get(), p.nx, p.ny, p.xy_plane))
             return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane needs nx == ny with a plane kernel that fits LDS "
                                                         "(16, 32, 48, 64; 128 except the FP64 xy-plane kernel)");
         c->use_plane = p.xy_plane > 0 ? p.xy_plane : 0;  // auto: decided by scft_autotune
@@ -1325,11 +1355,9 @@ static hipError_t launch_unit(scft_ctx *c, int unit, const PassArgs<T> &pa, bool
         const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
         const long np = (long)c->batch * c->n[2];
         const C *twy = (const C *)c->tw[1];
-        if (fused) return launch_plane_fused<T>(c->use_plane, c->n[0], pa, twy, kfy, np, c->stream);
-        return launch_plane_form<T>(c->use_plane, c->n[0], pa, twy, kfy, np, c->stream);
+        return run_plane<T>(c, c->use_plane, pa, twy, kfy, np, fused);
     }
-    if (fused) return launch_fused<T>(unit, c->n[unit], c->pass_variant[unit], pa, c->stream);
-    return launch_pass<T>(unit, c->n[unit], c->pass_variant[unit], pa, c->stream);
+    return run_pass<T>(c, unit, c->n[unit], c->pass_variant[unit], pa, fused);
 }
 
 // the units of a step in forward order (the xy-plane kernel counts as one)
@@ -1863,10 +1891,9 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     const long np = (long)c->batch * c->n[2];
     for (int r = 0; r < reps; ++r) {
         const C *twy = (const C *)c->tw[1];
-        hipError_t e = form && fused ? launch_plane_fused<T>(form, c->n[0], pa, twy, kfy, np, c->stream)
-                       : form        ? launch_plane_form<T>(form, c->n[0], pa, twy, kfy, np, c->stream)
-                       : fused       ? launch_fused<T>(ax, c->n[ax], variant, pa, c->stream)
-                                     : launch_pass<T>(axis, c->n[axis], variant, pa, c->stream);
+        hipError_t e = form    ? run_plane<T>(c, form, pa, twy, kfy, np, fused)
+                       : fused ? run_pass<T>(c, ax, c->n[ax], variant, pa, true)
+                               : run_pass<T>(c, axis, c->n[axis], variant, pa, false);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1946,7 +1973,7 @@ extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
     if (!ctx || !ms || reps < 1 || axis < 0 || axis > 11) return SCFT_E_INVALID;
     const int form = (axis == 3 || axis == 8) ? 1 : (axis == 4 || axis == 9) ? 2 : (axis == 10 || axis == 11) ? 3 : 0;
-    if (form && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form))
+    if (form && !plane_fits(ctx, ctx->n[0], ctx->n[1], form))
         return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel of this form for this grid");
     const int line_axis = axis < 3 ? axis : (axis >= 5 && axis <= 7 ? axis - 5 : -1);
     if (line_axis >= 0 && !ctx->active[line_axis]) return set_err(ctx, SCFT_E_INVALID, "axis has length 1 (no pass)");
@@ -1993,7 +2020,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     if (ctx->p.xy_plane < 0) {
         float best = 0.0f;
         for (int form = 0; form <= 3; ++form) {
-            if (form > 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], form)) continue;
+            if (form > 0 && !plane_fits(ctx, ctx->n[0], ctx->n[1], form)) continue;
             ctx->use_plane = form;
             float t = 0.0f;
             int rc = time_march(ctx, 2, &t);
@@ -2063,7 +2090,7 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
         if (choice[a] < 0 || choice[a] >= PASS_VARIANTS)
             return set_err(ctx, SCFT_E_INVALID, "tiling variant must be 0.." + std::to_string(PASS_VARIANTS - 1));
     if (choice[3] < 0 || choice[3] > 3) return set_err(ctx, SCFT_E_INVALID, "plane form must be 0..3");
-    if (choice[3] != 0 && !plane_fits(ctx->fp32, ctx->n[0], ctx->n[1], choice[3]))
+    if (choice[3] != 0 && !plane_fits(ctx, ctx->n[0], ctx->n[1], choice[3]))
         return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
     for (int a = 0; a < 3; ++a) {  // one trial launch: a variant whose LDS does not fit fails here
         if (!ctx->active[a] || choice[a] == 0) continue;

@@ -1,0 +1,115 @@
+"""GPU tests of the FP32X form: the propagator stored, loaded and stored back in
+FP32 (the HBM bytes of the FP32 form), every transform computed in FP64
+registers and LDS, Q, phi, F and the fields in FP64.  Its error against the
+FP64 oracle (or the FP64 form, which equals the oracle to ~1e-12) comes from
+one FP32 rounding per stored value.  Those roundings are independent from
+point to point, so Q's error averages over the grid: relQ * sqrt(N) ~ 2-10e-6
+(5e-8 at 32^3, 7e-9 at 64^3, 1.4e-9 at 128^3), where the FP32 form's FFT
+arithmetic leaves a systematic ~1e-7 at every size.  The pointwise phi error
+is the same storage floor in both forms (~1e-6 after Ns = 200 steps).  The
+bounds are written in each test, and the north-star bar |dF| < 1e-7 is
+checked at the full 64^3 and 128^3 sizes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32, FP32X = 0, 1, 2
+
+
+def _params(pkg, n, L, chiN, fA, ds, prec, **kw):
+    nx, ny, nz = n if isinstance(n, tuple) else (n, n, n)
+    return pkg.default_params(nx=nx, ny=ny, nz=nz, lx=L, ly=L, lz=L, chiN=chiN, fA=fA, ds=ds, precision=prec, **kw)
+
+
+def _oparams(orc, p):
+    return orc.params(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, p.chiN, p.fA, p.ds, pi=p.pi, wopt=p.wopt, wcmp=p.wcmp)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64)])
+def test_fp32x_single_conc_vs_oracle(pkg, orc, shape):
+    """One getConc on the FP32 test's fields: FP32X's Q within 2e-5/sqrt(N) of
+    the oracle and closer than the FP32 form's; phi at the storage floor."""
+    print("case", shape, flush=True)
+    err = {}
+    for prec in (FP32, FP32X):
+        p = _params(pkg, shape, 4.2, 20.0, 0.3, 0.005, prec)
+        wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "hexagonal", 1.0, 0.05, 2)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations()[0]
+            phiA, phiB, _, _ = s.get_fields()
+        op = _oparams(orc, p)
+        pa, pb, Qo = orc.conc(op, orc.ktable(op), wA.copy(), wB.copy())
+        err[prec] = (abs(Q - Qo) / Qo, float(np.max(np.abs(phiA[0] - pa))), float(np.max(np.abs(phiB[0] - pb))))
+        print(f"  prec={prec}: relQ={err[prec][0]:.3e} max|dphiA|={err[prec][1]:.3e} max|dphiB|={err[prec][2]:.3e}")
+    eQ, eA, eB = err[FP32X]
+    N = shape[0] * shape[1] * shape[2]
+    assert eQ < 2e-5 / np.sqrt(N) and eA < 1e-6 and eB < 3e-6
+    assert eQ < err[FP32][0]
+
+
+def test_fp32x_c1_iterations_vs_oracle(pkg, orc):
+    """20 simple-mixing iterations of c1 (32^3): F and phi against the oracle."""
+    p = _params(pkg, 32, 4.0, 15.0, 0.5, 0.01, FP32X)
+    wA, wB = pkg.fields.workload_fields("c1_lamellar_32")
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        _, fe = s.iterate(20)
+        phiA, _, _, _ = s.get_fields()
+    op = _oparams(orc, p)
+    pa, pb, hist = orc.run(op, wA.copy(), wB.copy(), 20)
+    dF = abs(fe[0][0] - hist[-1][0])
+    dphi = float(np.max(np.abs(phiA[0] - pa)))
+    print(f"FP32X c1 20 it: dF={dF:.3e} max|dphi|={dphi:.3e}")
+    assert dF < 1e-7 and dphi < 1e-6
+
+
+@pytest.mark.parametrize("name", ["c2_cylinder_64", "c3_gyroid_128"])
+def test_fp32x_full_size_single_eval(pkg, name):
+    """getConc, update and F at the full 64^3 / 128^3, Ns = 200 workloads on
+    their seeded fields: |dF| < 1e-7 against the FP64 form."""
+    print("case", name, flush=True)
+    w = pkg.fields.WORKLOADS[name]
+    wA, wB = pkg.fields.workload_fields(name)
+    out = {}
+    for prec in (FP64, FP32X):
+        p = _params(pkg, w["n"], w["L"], w["chiN"], w["fA"], w["ds"], prec, mixing=1, anderson_m=w["anderson_m"],
+                    anderson_mix=0.1)
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            Q = s.concentrations()[0]
+            phiA, _, _, _ = s.get_fields()
+            s.update_fields()
+            out[prec] = (Q, s.free_energy()[0][0], phiA[0].copy())
+    (q64, f64, a64), (q, f, a) = out[FP64], out[FP32X]
+    eQ, dF, dphi = abs(q - q64) / q64, abs(f - f64), float(np.max(np.abs(a - a64)))
+    print(f"  relQ={eQ:.3e} dF={dF:.3e} max|dphiA|={dphi:.3e}")
+    assert dF < 1e-7 and eQ < 2e-5 / np.sqrt(w["n"] ** 3) and dphi < 5e-7
+
+
+def test_fp32x_adjoint_identity_and_gradient(pkg):
+    """The chain adjoint in FP32X: dlnQ/du = -phi/N, and the loss and gradient
+    through 5 Anderson iterations against FP64 (the gradient is pointwise, so
+    it sits at the storage floor: ~1e-3 of its maximum on this 16^3 grid)."""
+    p = _params(pkg, (16, 16, 16), 4.0, 18.0, 0.5, 0.01, FP32X)
+    N = 4096
+    wA, wB = pkg.fields.seeded_fields(16, 16, 16, 4.0, 4.0, 4.0, "lamellar", 1.5, 0.2, 72)
+    with pkg.Solver(p) as s:
+        L, gA, gB, pa, pb, Q = s.conc_adjoint(wA, wB, None, None, 1.0)
+    eA = float(np.max(np.abs(gA[0] + pa[0] / N)) / np.max(np.abs(pa[0] / N)))
+    print(f"rel |dlnQ/du + phi/N| = {eA:.2e}")
+    assert eA < 1e-6
+    res = {}
+    x = np.arange(N) % 16
+    target = 0.5 + 0.3 * np.cos(2 * np.pi * x / 16)
+    h = 0.01 * np.sin(2 * np.pi * x / 16)
+    for prec in (FP64, FP32X):
+        q = _params(pkg, (16, 16, 16), 4.0, 18.0, 0.5, 0.01, prec, mixing=1, anderson_m=3, anderson_mix=0.2)
+        with pkg.Solver(q) as s:
+            res[prec] = s.adjoint_loop(h, wA, wB, 5, target)
+    dl = abs(res[FP32X][0][0] - res[FP64][0][0]) / abs(res[FP64][0][0])
+    g64 = res[FP64][1][0]
+    dg = float(np.max(np.abs(res[FP32X][1][0] - g64)) / np.max(np.abs(g64)))
+    print(f"FP32X vs FP64 loop adjoint: rel dloss={dl:.2e} rel dgrad={dg:.2e}")
+    assert dl < 1e-5 and dg < 3e-3

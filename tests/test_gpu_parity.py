@@ -145,7 +145,7 @@ def test_disordered_kat_full_size(pkg, n, fA, ds, prec):
 
 
 @pytest.mark.parametrize("mixing", [0, 1])
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_graph_replay_matches_eager(pkg, mixing, prec):
     """The captured-graph iteration equals the eager launch sequence bit for bit,
     replay after replay, with eager copies and host syncs between replays."""
@@ -237,14 +237,16 @@ def test_q_underflow_reported(pkg):
 
 @pytest.mark.parametrize("shape,prec", [((32, 32, 32), 0), ((64, 64, 8), 0), ((48, 48, 6), 1),
                                         ((128, 128, 4), 1), ((128, 128, 4), 0), ((64, 64, 1), 0),
-                                        ((16, 16, 24), 1), ((16, 16, 24), 0)])
+                                        ((16, 16, 24), 1), ((16, 16, 24), 0), ((64, 64, 8), 2), ((128, 128, 4), 2),
+                                        ((48, 48, 6), 2)])
 def test_xy_plane_kernel(pkg, orc, shape, prec):
     """Both xy-plane kernels (1: one workgroup per plane, 2: split into the
     real and imaginary parts) equal the separate-pass march and the oracle."""
     print("case", shape, prec, flush=True)
     out = {}
     # the FP64 128^2 plane fits LDS only in the two-level split form
-    forms = [f for f in (0, 1, 2, 3) if not (f in (1, 3) and prec == 0 and shape[0] == 128)
+    # (FP32X transforms in FP64 registers and LDS: the same plane limits as FP64)
+    forms = [f for f in (0, 1, 2, 3) if not (f in (1, 3) and prec != 1 and shape[0] == 128)
              and not (f == 3 and shape[0] not in (32, 64, 128))]
     for plane in forms:
         p = _params(pkg, shape, (4.0, 4.0, 3.0), 16.0, 0.35, 0.02, prec=prec, xy_plane=plane)
@@ -256,7 +258,7 @@ def test_xy_plane_kernel(pkg, orc, shape, prec):
         out[plane] = (Q, phiA)
     op = _oparams(orc, p)
     pa, pb, Qo = orc.conc(op, orc.ktable(op), wA.copy(), wB.copy())
-    tol = 1e-12 if prec == 0 else 2e-5
+    tol = {0: 1e-12, 1: 2e-5, 2: 1e-6}[prec]
     for plane in forms:
         eQ, ephi = abs(out[plane][0] - Qo) / Qo, _rel(out[plane][1], pa)
         print(f"  plane={plane}: relQ={eQ:.3e} rel_phiA={ephi:.3e}")
@@ -267,7 +269,9 @@ def test_xy_plane_kernel(pkg, orc, shape, prec):
     (1, (32, 32, 32), 0), (1, (24, 20, 36), 1), (1, (64, 64, 8), 1), (1, (2, 10, 6), 1), (1, (128, 16, 12), 1),
     (1, (20, 36, 8), 0),
     # three-level tiling on every axis of length 32, 64, 128, 256 (others keep the default)
-    (2, (32, 32, 32), 0), (2, (64, 32, 128), 0), (2, (128, 64, 8), 1), (2, (256, 4, 6), 0), (2, (64, 64, 64), 1)])
+    (2, (32, 32, 32), 0), (2, (64, 32, 128), 0), (2, (128, 64, 8), 1), (2, (256, 4, 6), 0), (2, (64, 64, 64), 1),
+    # FP32X: FP64 transforms over FP32 storage
+    (1, (24, 20, 36), 2), (1, (64, 64, 8), 2), (2, (128, 64, 8), 2), (2, (64, 32, 128), 2)])
 def test_tiling_variants_identical(pkg, shape, prec, tiling):
     """Every pass tiling variant does the same arithmetic per line (up to the
     compiler's FMA contraction choices): equal to the default to rounding,
@@ -280,7 +284,7 @@ def test_tiling_variants_identical(pkg, shape, prec, tiling):
         with pkg.Solver(p) as s:
             s.set_fields(wA, wB)
             res.append((s.concentrations()[0], s.get_fields()[0]))
-    tol = 1e-13 if prec == 0 else 1e-6
+    tol = {0: 1e-13, 1: 1e-6, 2: 1e-7}[prec]
     assert abs(res[0][0] - res[1][0]) / res[0][0] < tol
     assert _rel(res[1][1], res[0][1]) < tol
 
@@ -351,7 +355,8 @@ def test_split_plane_batched_iterations(pkg, prec):
 @pytest.mark.parametrize("shape,plane,prec,ckpt", [
     ((32, 32, 32), 0, 0, 0), ((32, 32, 32), 1, 0, 0), ((32, 32, 32), 2, 0, 0), ((16, 24, 8), 0, 0, 0),
     ((24, 20, 1), 0, 0, 0), ((16, 16, 1), 2, 0, 0), ((32, 32, 32), 2, 1, 0), ((32, 32, 32), 0, 0, 1),
-    ((64, 64, 16), 1, 1, 1), ((64, 64, 16), 3, 0, 0), ((32, 32, 8), 3, 1, 1), ((128, 128, 4), 3, 1, 0)])
+    ((64, 64, 16), 1, 1, 1), ((64, 64, 16), 3, 0, 0), ((32, 32, 8), 3, 1, 1), ((128, 128, 4), 3, 1, 0),
+    ((32, 32, 32), 0, 2, 0), ((64, 64, 16), 3, 2, 0), ((32, 32, 8), 2, 2, 1), ((128, 128, 4), 2, 2, 0)])
 def test_fused_march_matches_unfused(pkg, orc, shape, plane, prec, ckpt):
     """The march with step-boundary kernels (alternating axis order, one
     kernel per boundary) equals the step-by-step march: FP64 to rounding (the
@@ -371,7 +376,7 @@ def test_fused_march_matches_unfused(pkg, orc, shape, plane, prec, ckpt):
             Q = s.concentrations().copy()
             phiA = s.get_fields()[0].copy()
         res[fuse] = (Q, phiA)
-    tol = 1e-12 if prec == 0 else 2e-5
+    tol = {0: 1e-12, 1: 2e-5, 2: 1e-6}[prec]
     assert np.max(np.abs(res[1][0] - res[0][0]) / res[0][0]) < tol
     assert _rel(res[1][1], res[0][1]) < tol
     if prec == 0:

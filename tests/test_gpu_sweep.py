@@ -11,7 +11,7 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1), (0, 2)])
 def test_sweep_matches_single_instances(pkg, mixing, prec):
     inst = pkg.fields.sweep_instances(count=9, seed=11, sizes=(16, 24, 32))
     fields = pkg.fields.sweep_fields(inst, noise=0.2, seed=500)
@@ -36,7 +36,8 @@ def test_sweep_matches_single_instances(pkg, mixing, prec):
         # instance's defaults: FP64 rounding differences, which 6 iterations of Anderson's least squares
         # (m = 3 from noise) amplify about 1e5-fold; simple mixing keeps them at rounding level
         # (FP32: the kernel forms' own rounding, ~1e-7 per evaluation, over 6 iterations)
-        tol = (1e-12 if mixing == 0 else 1e-9) if prec == 0 else 5e-6
+        # (FP32X: one FP32 rounding per stored value, whose order the kernel forms change)
+        tol = (1e-12 if mixing == 0 else 1e-9) if prec == 0 else (5e-6 if prec == 1 else 2e-6)
         assert np.max(np.abs(fe[i] - fe1[0])) < tol * max(1, np.max(np.abs(fe1[0]))), (i, fe[i], fe1[0])
         assert _rel(phis[i], phi1) < tol, i
 
