@@ -103,8 +103,16 @@ int read_config(const std::string &path, Config &c, std::string &err)
         trim(key);
         trim(val);
         const char *vs = val.c_str();
-        if (key == "precision") c.precision = (!strcasecmp(vs, "fp32") || !strcasecmp(vs, "float")) ? SCFT_FP32 : SCFT_FP64;
-        else if (key == "mixing") c.mixing = !strcasecmp(vs, "anderson") ? SCFT_MIX_ANDERSON : SCFT_MIX_SIMPLE;
+        if (key == "precision") {
+            if (!strcasecmp(vs, "fp64") || !strcasecmp(vs, "double")) c.precision = SCFT_FP64;
+            else if (!strcasecmp(vs, "fp32") || !strcasecmp(vs, "float")) c.precision = SCFT_FP32;
+            else if (!strcasecmp(vs, "fp32x")) c.precision = SCFT_FP32X;
+            else {
+                std::fclose(fp);
+                err = path + ": precision must be fp64, fp32 or fp32x, not '" + val + "'";
+                return SCFT_E_IO;
+            }
+        } else if (key == "mixing") c.mixing = !strcasecmp(vs, "anderson") ? SCFT_MIX_ANDERSON : SCFT_MIX_SIMPLE;
         else if (key == "anderson_m") c.anderson_m = std::atoi(vs);
         else if (key == "anderson_mix") c.anderson_mix = std::atof(vs);
         else if (key == "wopt") c.wopt = std::atof(vs);

@@ -115,3 +115,15 @@ def test_driver_run_matches_library(pkg, tmp_path):
         _, fe = s.iterate(20)
     assert np.allclose(vals, fe[0], rtol=5e-8, atol=1e-12)  # %.8e printing
     assert not os.path.exists(tmp_path / "pha_1.dat")  # written every 100 iterations
+
+
+def test_precision_key(tmp_path):
+    """precision= takes fp64/double, fp32/float and fp32x; anything else is an
+    input error naming the key, not a silent FP64 run."""
+    for v in ("fp64", "double", "fp32", "float", "fp32x"):
+        write_ab(tmp_path / "ab.txt", extra=f"precision={v}\n")
+        r = run(["-gen"], str(tmp_path))
+        assert r.returncode == 0, (v, r.stdout)
+    write_ab(tmp_path / "ab.txt", extra="precision=half\n")
+    r = run(["-gen"], str(tmp_path))
+    assert r.returncode != 0 and "precision must be fp64, fp32 or fp32x" in r.stdout, r.stdout
