@@ -57,13 +57,14 @@ def workload_params(pkg, name, prec, **kw):
 
 
 def convergence(pkg, name, n_iter, choices):
-    """Both forms from the same seeded fields for n_iter iterations (ridge
-    ANDERSON_RIDGE): F and the incompressibility error max|1 - phiA - phiB|
-    at iterations 10, 25, 50, 100, ... and |dF| between the forms."""
+    """The FP64, FP32 and FP32X forms from the same seeded fields for n_iter
+    iterations (ridge ANDERSON_RIDGE): F and the incompressibility error
+    max|1 - phiA - phiB| at iterations 10, 25, 50, 100, ... and |dF| of each
+    FP32-storage form against FP64."""
     wA, wB = pkg.fields.workload_fields(name)
     marks = [k for k in (10, 25, 50, 100, 200, 400) if k <= n_iter]
     out = {}
-    for prec, ch in ((pkg.FP64, choices[0]), (pkg.FP32, choices[1])):
+    for prec, ch in ((pkg.FP64, choices[0]), (pkg.FP32, choices[1]), (pkg.FP32X, choices[2])):
         with pkg.Solver(workload_params(pkg, name, prec)) as s:
             s.set_tiling(ch)
             s.set_fields(wA, wB)
@@ -76,7 +77,9 @@ def convergence(pkg, name, n_iter, choices):
     return {"iterations": marks, "ridge": ANDERSON_RIDGE,
             "F_fp64": [out[pkg.FP64][k][0] for k in marks], "err_fp64": [out[pkg.FP64][k][1] for k in marks],
             "F_fp32": [out[pkg.FP32][k][0] for k in marks], "err_fp32": [out[pkg.FP32][k][1] for k in marks],
-            "dF": [abs(out[pkg.FP32][k][0] - out[pkg.FP64][k][0]) for k in marks]}
+            "F_fp32x": [out[pkg.FP32X][k][0] for k in marks], "err_fp32x": [out[pkg.FP32X][k][1] for k in marks],
+            "dF": [abs(out[pkg.FP32][k][0] - out[pkg.FP64][k][0]) for k in marks],
+            "dF_fp32x": [abs(out[pkg.FP32X][k][0] - out[pkg.FP64][k][0]) for k in marks]}
 
 
 def iter_bytes(N, Ns, b):
@@ -392,8 +395,8 @@ def main():
         forms["c3_128"]["single_eval_seeded_fields"] = {
             "fp32": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3)),
             "fp32x": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3), pkg.FP32X)}
-        log("[bench] c3 100 iterations of both forms from the seeded fields")
-        forms["c3_128"]["convergence"] = convergence(pkg, name3, 100, (t64.tuning[0], t32.tuning[0]))
+        log("[bench] c3 200 iterations of the three forms from the seeded fields")
+        forms["c3_128"]["convergence"] = convergence(pkg, name3, 200, (t64.tuning[0], t32.tuning[0], tx.tuning[0]))
         log("[bench] c3 FP32 with sqrt(Ns) checkpointing")
         tc, mc, _, _ = run_form(pkg, name3, pkg.FP32, 4, 1, checkpoint=1)
         forms["c3_128"]["fp32_ckpt_iters_per_s"] = 1e3 / mc
