@@ -145,9 +145,9 @@ def c5_adjoint(pkg, n_dirs=2):
     """c5: loss = ||phiA - phi_target||^2 after 50 unrolled Anderson iterations
     (64^3, f = 0.5, chiN = 18, ds = 0.01, sqrt(Ns) checkpointing), gradient with
     respect to an external field h on A by the discrete adjoint.  Reports the
-    time per gradient in both forms, central differences (eps = 1e-5) along
-    random directions of the FP64 gradient, and the FP32 gradient's distance
-    from the FP64 one.  Anderson m = 10 with the relative ridge 1e-2 on its
+    time per gradient in the FP64, FP32 and FP32X forms, central differences
+    (eps = 1e-5) along random directions of the FP64 gradient, and the FP32
+    and FP32X gradients' distance from the FP64 one.  Anderson m = 10 with the relative ridge 1e-2 on its
     least squares: without it the 50-iteration loss is not smooth on the
     finite-difference scale (the least squares turns ill-conditioned)."""
     w = pkg.fields.WORKLOADS["c5_adjoint_64"]
@@ -160,7 +160,7 @@ def c5_adjoint(pkg, n_dirs=2):
     h = np.zeros(N)
     out = {}
     grads = {}
-    for prec in (pkg.FP64, pkg.FP32):
+    for prec in (pkg.FP64, pkg.FP32, pkg.FP32X):
         p = pkg.default_params(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, chiN=w["chiN"], fA=w["fA"], ds=w["ds"],
                                precision=prec, mixing=pkg.MIX_ANDERSON, anderson_m=10, anderson_mix=0.1,
                                checkpoint=1, anderson_reg=1e-2)
@@ -170,7 +170,7 @@ def c5_adjoint(pkg, n_dirs=2):
             t0 = time.perf_counter()
             loss, g, _ = s.adjoint_loop(h, w0A, w0B, w["iters"], target)
             dt = time.perf_counter() - t0
-            key = "fp32" if prec else "fp64"
+            key = {pkg.FP64: "fp64", pkg.FP32: "fp32", pkg.FP32X: "fp32x"}[prec]
             out[f"{key}_s_per_gradient"] = dt
             out[f"{key}_loss"] = float(loss[0])
             grads[key] = g[0].copy()
@@ -187,7 +187,8 @@ def c5_adjoint(pkg, n_dirs=2):
                     worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
                 out["fp64_fd_rel_mismatch"] = worst
                 out["fd_directions"] = n_dirs
-    out["fp32_vs_fp64_grad_rel"] = float(np.max(np.abs(grads["fp32"] - grads["fp64"])) / np.max(np.abs(grads["fp64"])))
+    for key in ("fp32", "fp32x"):
+        out[f"{key}_vs_fp64_grad_rel"] = float(np.max(np.abs(grads[key] - grads["fp64"])) / np.max(np.abs(grads["fp64"])))
     out["iterations"] = w["iters"]
     out["anderson"] = {"m": 10, "mix": 0.1, "ridge": 1e-2}
     return out
