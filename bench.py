@@ -136,9 +136,11 @@ KERNEL_NAMES = {"x": "k_line_conv x pass", "y": "k_line_conv y pass", "z": "k_li
                 "fused_x": "k_line_fused x boundary", "fused_y": "k_line_fused y boundary",
                 "fused_z": "k_line_fused z boundary", "fused_plane": "k_plane_xy boundary form",
                 "fused_split": "k_plane_split boundary form",
-                "fused_split3": "k_plane_split boundary form, three-level tiling"}
+                "fused_split3": "k_plane_split boundary form, three-level tiling",
+                "fused_split_y": "k_plane_split boundary form, lane-local y lines",
+                "fused_split3_y": "k_plane_split boundary form, three-level tiling, lane-local y lines"}
 CODES = {"x": 0, "y": 1, "z": 2, "xy_plane": 3, "xy_split": 4, "fused_x": 5, "fused_y": 6, "fused_z": 7,
-         "fused_plane": 8, "fused_split": 9, "fused_split3": 11}
+         "fused_plane": 8, "fused_split": 9, "fused_split3": 11, "fused_split_y": 13, "fused_split3_y": 15}
 
 
 def c5_adjoint(pkg, n_dirs=2):
@@ -213,7 +215,8 @@ def pass_roofline(s, prec, reps=50):
     if plane == 0:
         kernels = {"y": 1.0, "fused_x": 0.5, "fused_z": 0.5}
     else:
-        kernels = {{1: "fused_plane", 2: "fused_split", 3: "fused_split3"}[plane]: 0.5, "fused_z": 0.5}
+        kernels = {{1: "fused_plane", 2: "fused_split", 3: "fused_split3", 4: "fused_split_y",
+                    5: "fused_split3_y"}[plane]: 0.5, "fused_z": 0.5}
     times = {k: s.time_pass(CODES[k], reps) for k in kernels}
     share = {k: kernels[k] * times[k] for k in kernels}
     dom = max(share, key=share.get)

@@ -75,8 +75,9 @@ typedef struct {
     int xy_plane;         /* -1 auto (scft_autotune decides), 0 separate x and y passes,
                              1 fused xy-plane kernel (one workgroup per plane), 2 split
                              xy-plane kernel (one workgroup per plane and real part), 3 the
-                             split kernel with the three-level line tiling (n = 32, 64, 128);
-                             1..3 need nx == ny and a plane that fits LDS */
+                             split kernel with the three-level line tiling (n = 32, 64, 128),
+                             4 and 5 forms 2 and 3 with each y line on consecutive lanes of
+                             one wave; 1..5 need nx == ny and a plane that fits LDS */
     int tiling;           /* -1 default (one line per thread group until scft_autotune),
                              0 or 1 force one or two lines per thread group on every pass,
                              2 the three-level tiling (2x the threads per line; n = 32, 64,
@@ -146,9 +147,9 @@ int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, double *fe);
 int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
 
 /* Kernel-level timing for the roofline: runs `reps` back-to-back launches of
-   the propagator axis pass `axis` (0 x, 1 y, 2 z; 3, 4, 10 the xy-plane
-   kernel forms 1, 2, 3; 5, 6, 7 the step-boundary x, y, z kernels; 8, 9, 11
-   the step-boundary forms of 3, 4, 10)
+   the propagator axis pass `axis` (0 x, 1 y, 2 z; 3, 4, 10, 12, 14 the
+   xy-plane kernel forms 1, 2, 3, 4, 5; 5, 6, 7 the step-boundary x, y, z
+   kernels; 8, 9, 11, 13, 15 the step-boundary forms of 3, 4, 10, 12, 14)
    on a copy of slice 0
    between two HIP events and returns the mean duration in ms.  Leaves the
    solver state unchanged except the scratch field. */
@@ -182,7 +183,8 @@ int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const d
    by factor_decompose_1024, init_cuda.cu:21-80): times every tiling variant of
    each axis pass (`reps` launches each) and the fused xy-plane kernel, keeps
    the fastest.  choice (may be NULL): int[4] = variant of x, y, z pass (0..2),
-   plane form used (0 none, 1 xy-plane, 2 split xy-plane, 3 split three-level);
+   plane form used (0 none, 1 xy-plane, 2 split xy-plane, 3 split three-level,
+   4 and 5 the split forms with lane-local y lines);
    ms (may be NULL): float[4] = best x, y, z pass and chosen plane-kernel times.
    The plane form and the final axis variants are chosen by timing whole
    marches on stand-in factors, which overwrites the W factors and the slice
@@ -191,7 +193,7 @@ int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const d
 int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 
 /* Apply a tiling choice of the form scft_autotune returns (int[4]: variant of
-   the x, y, z pass, plane form 0..3) without timing anything, e.g. one
+   the x, y, z pass, plane form 0..5) without timing anything, e.g. one
    cached from an earlier scft_autotune on the same grid.  SCFT_E_UNSUPPORTED if
    a variant or the plane kernel does not fit this grid.  No reference
    counterpart (its launch shapes are fixed, init_cuda.cu:21-80). */

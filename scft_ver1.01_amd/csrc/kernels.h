@@ -70,7 +70,8 @@ template <typename T>
 hipError_t launch_pass(int axis, int n, int variant, const PassArgs<T> &a, hipStream_t s);
 // xy-plane kernels for nx = ny = n.  form 1: one workgroup per plane; form 2:
 // one workgroup per plane and real part; form 3: form 2 with the three-level
-// tiling.  nplanes = batch * nz.
+// tiling; forms 4 and 5: forms 2 and 3 with each y line on consecutive lanes
+// of one wave.  nplanes = batch * nz.
 template <typename T>
 bool plane_form_supported(int form, int n);
 template <typename T>

@@ -978,9 +978,27 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
 // true if xy-plane form `form` (1 xy-plane, 2 split xy-plane, 3 split with the
 // three-level tiling) has a kernel for
 // an nx x ny plane in this precision that fits a CU's LDS
+static constexpr int PLANE_FORMS = 5;  // xy-plane forms 1..5 (0: separate x and y passes)
+
+// pass-timing codes (scft_time_pass): 0..2 the axis passes, 5..7 their
+// step-boundary kernels; xy-plane form f has a pass code and a boundary code
+static constexpr int kPlaneCode[PLANE_FORMS + 1][2] = {{-1, -1}, {3, 8}, {4, 9}, {10, 11}, {12, 13}, {14, 15}};
+static constexpr int PASS_CODES = 16;
+static void code_form(int code, int *form, bool *fused)
+{
+    *form = 0;
+    *fused = code >= 5 && code <= 7;
+    for (int f = 1; f <= PLANE_FORMS; ++f)
+        for (int k = 0; k < 2; ++k)
+            if (kPlaneCode[f][k] == code) {
+                *form = f;
+                *fused = k == 1;
+            }
+}
+
 static bool plane_fits(const scft_ctx *c, int nx, int ny, int form)
 {
-    if (nx != ny || nx < 2 || form < 1 || form > 3) return false;
+    if (nx != ny || nx < 2 || form < 1 || form > PLANE_FORMS) return false;
     if (c->mixed) return scft_f32x::plane_form_supported<float>(form, nx);
     return c->fp32 ? plane_form_supported<float>(form, nx) : plane_form_supported<double>(form, nx);
 }
@@ -1116,7 +1134,8 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     }
     // reduction blocks per instance: ~2 points per thread, at most 2048 partials
     {
-        if (p.xy_plane < -1 || p.xy_plane > 3) return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1..3");
+        if (p.xy_plane < -1 || p.xy_plane > PLANE_FORMS)
+            return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1.." + std::to_string(PLANE_FORMS));
         if (!(p.anderson_reg >= 0.0)) return set_err(nullptr, SCFT_E_INVALID, "anderson_reg must be >= 0");
         if (p.xy_plane > 0 && !plane_fits(c.get(), p.nx, p.ny, p.xy_plane))
             return set_err(nullptr, SCFT_E_UNSUPPORTED, "xy_plane needs nx == ny with a plane kernel that fits LDS "
@@ -1865,10 +1884,10 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.dst = (C *)c->work;
     pa.W = (const T *)c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
-    // codes: 0..2 axis pass, 3/4/10 plane forms 1/2/3; 5..7 boundary axis
-    // kernels, 8/9/11 boundary plane forms 1/2/3 (these store z(t) to a scratch slice)
-    const bool fused = (axis >= 5 && axis <= 9) || axis == 11;
-    const int form = (axis == 3 || axis == 8) ? 1 : (axis == 4 || axis == 9) ? 2 : (axis == 10 || axis == 11) ? 3 : 0;
+    // codes: see kPlaneCode (the boundary kernels store z(t) to a scratch slice)
+    int form = 0;
+    bool fused = false;
+    code_form(axis, &form, &fused);
     pa.slot = fused ? slot_ptr<T>(c, 1) : nullptr;
     pa.wn[0] = pa.wn[1] = 0u;
     pa.post_scale = (T)(1.0 / (double)c->N);
@@ -1971,8 +1990,10 @@ static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, flo
 
 extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
-    if (!ctx || !ms || reps < 1 || axis < 0 || axis > 11) return SCFT_E_INVALID;
-    const int form = (axis == 3 || axis == 8) ? 1 : (axis == 4 || axis == 9) ? 2 : (axis == 10 || axis == 11) ? 3 : 0;
+    if (!ctx || !ms || reps < 1 || axis < 0 || axis >= PASS_CODES) return SCFT_E_INVALID;
+    int form = 0;
+    bool fused = false;
+    code_form(axis, &form, &fused);
     if (form && !plane_fits(ctx, ctx->n[0], ctx->n[1], form))
         return set_err(ctx, SCFT_E_INVALID, "no xy-plane kernel of this form for this grid");
     const int line_axis = axis < 3 ? axis : (axis >= 5 && axis <= 7 ? axis - 5 : -1);
@@ -2019,7 +2040,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     int plane = ctx->p.xy_plane > 0 ? ctx->p.xy_plane : 0;
     if (ctx->p.xy_plane < 0) {
         float best = 0.0f;
-        for (int form = 0; form <= 3; ++form) {
+        for (int form = 0; form <= PLANE_FORMS; ++form) {
             if (form > 0 && !plane_fits(ctx, ctx->n[0], ctx->n[1], form)) continue;
             ctx->use_plane = form;
             float t = 0.0f;
@@ -2060,7 +2081,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     }
     float tplane = 0.0f;
     if (plane) {
-        int rc = time_pass_variant(ctx, plane == 3 ? 10 : 2 + plane, reps, 0, &tplane);
+        int rc = time_pass_variant(ctx, kPlaneCode[plane][0], reps, 0, &tplane);
         if (rc) return rc;
     }
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -2089,7 +2110,8 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
     for (int a = 0; a < 3; ++a)
         if (choice[a] < 0 || choice[a] >= PASS_VARIANTS)
             return set_err(ctx, SCFT_E_INVALID, "tiling variant must be 0.." + std::to_string(PASS_VARIANTS - 1));
-    if (choice[3] < 0 || choice[3] > 3) return set_err(ctx, SCFT_E_INVALID, "plane form must be 0..3");
+    if (choice[3] < 0 || choice[3] > PLANE_FORMS)
+        return set_err(ctx, SCFT_E_INVALID, "plane form must be 0.." + std::to_string(PLANE_FORMS));
     if (choice[3] != 0 && !plane_fits(ctx, ctx->n[0], ctx->n[1], choice[3]))
         return set_err(ctx, SCFT_E_UNSUPPORTED, "xy-plane kernel does not fit this grid");
     for (int a = 0; a < 3; ++a) {  // one trial launch: a variant whose LDS does not fit fails here

@@ -240,14 +240,15 @@ def test_q_underflow_reported(pkg):
                                         ((16, 16, 24), 1), ((16, 16, 24), 0), ((64, 64, 8), 2), ((128, 128, 4), 2),
                                         ((48, 48, 6), 2)])
 def test_xy_plane_kernel(pkg, orc, shape, prec):
-    """Both xy-plane kernels (1: one workgroup per plane, 2: split into the
-    real and imaginary parts) equal the separate-pass march and the oracle."""
+    """Every xy-plane kernel form (1: one workgroup per plane; 2, 3: split into
+    the real and imaginary parts, two- and three-level tiling; 4, 5: the split
+    forms with lane-local y lines) equals the separate-pass march and the oracle."""
     print("case", shape, prec, flush=True)
     out = {}
     # the FP64 128^2 plane fits LDS only in the two-level split form
     # (FP32X transforms in FP64 registers and LDS: the same plane limits as FP64)
-    forms = [f for f in (0, 1, 2, 3) if not (f in (1, 3) and prec != 1 and shape[0] == 128)
-             and not (f == 3 and shape[0] not in (32, 64, 128))]
+    forms = [f for f in (0, 1, 2, 3, 4, 5) if not (f in (1, 3, 5) and prec != 1 and shape[0] == 128)
+             and not (f in (3, 5) and shape[0] not in (32, 64, 128))]
     for plane in forms:
         p = _params(pkg, shape, (4.0, 4.0, 3.0), 16.0, 0.35, 0.02, prec=prec, xy_plane=plane)
         wA, wB = pkg.fields.seeded_fields(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, "gyroid", 1.2, 0.2, 51)
@@ -323,6 +324,8 @@ def test_set_tiling_matches_autotune_choice(pkg):
         _, fe1 = s.iterate(3)
         with pytest.raises(pkg.ScftError):
             s.set_tiling([0, 0, 9, 0])
+        with pytest.raises(pkg.ScftError):
+            s.set_tiling([0, 0, 0, 6])  # plane forms are 0..5
     print("choice", choice, flush=True)
     assert np.array_equal(fe0, fe1)
 
@@ -356,7 +359,9 @@ def test_split_plane_batched_iterations(pkg, prec):
     ((32, 32, 32), 0, 0, 0), ((32, 32, 32), 1, 0, 0), ((32, 32, 32), 2, 0, 0), ((16, 24, 8), 0, 0, 0),
     ((24, 20, 1), 0, 0, 0), ((16, 16, 1), 2, 0, 0), ((32, 32, 32), 2, 1, 0), ((32, 32, 32), 0, 0, 1),
     ((64, 64, 16), 1, 1, 1), ((64, 64, 16), 3, 0, 0), ((32, 32, 8), 3, 1, 1), ((128, 128, 4), 3, 1, 0),
-    ((32, 32, 32), 0, 2, 0), ((64, 64, 16), 3, 2, 0), ((32, 32, 8), 2, 2, 1), ((128, 128, 4), 2, 2, 0)])
+    ((32, 32, 32), 0, 2, 0), ((64, 64, 16), 3, 2, 0), ((32, 32, 8), 2, 2, 1), ((128, 128, 4), 2, 2, 0),
+    ((64, 64, 16), 4, 0, 0), ((128, 128, 4), 4, 2, 0), ((32, 32, 8), 5, 1, 1), ((64, 64, 16), 5, 2, 0),
+    ((128, 128, 4), 4, 0, 1), ((48, 48, 6), 4, 1, 0)])
 def test_fused_march_matches_unfused(pkg, orc, shape, plane, prec, ckpt):
     """The march with step-boundary kernels (alternating axis order, one
     kernel per boundary) equals the step-by-step march: FP64 to rounding (the
