@@ -139,6 +139,7 @@ KERNEL_NAMES = {"x": "k_line_conv x pass", "y": "k_line_conv y pass", "z": "k_li
                 "fused_split3": "k_plane_split boundary form, three-level tiling",
                 "fused_split_y": "k_plane_split boundary form, lane-local y lines",
                 "fused_split3_y": "k_plane_split boundary form, three-level tiling, lane-local y lines"}
+CONTOUR_CODE = 16  # scft_time_pass code of the contour kernel
 CODES = {"x": 0, "y": 1, "z": 2, "xy_plane": 3, "xy_split": 4, "fused_x": 5, "fused_y": 6, "fused_z": 7,
          "fused_plane": 8, "fused_split": 9, "fused_split3": 11, "fused_split_y": 13, "fused_split3_y": 15}
 
@@ -222,8 +223,12 @@ def pass_roofline(s, prec, reps=50):
     dom = max(share, key=share.get)
     bpl = {k: (4 * b * N if not k.startswith("fused") else (8 if post_exact else 10) * b * N) for k in kernels}
     achieved = bpl[dom] / (times[dom] * 1e-3) / 1e9
+    # the contour kernel (once per iteration): reads every slice once (2 b N (Ns + 1)) and writes phiA, phiB
+    tc = s.time_pass(CONTOUR_CODE, max(3, reps // 5))
+    bc = 2 * b * N * (s.NsA + s.NsB + 1) + 16 * N
     return {"dominant": dom, "kernel": KERNEL_NAMES[dom], "ms_per_launch": times, "step_share_ms": share,
-            "bytes_per_launch": bpl, "achieved": achieved, "grid": n}
+            "bytes_per_launch": bpl, "achieved": achieved, "grid": n,
+            "contour": {"ms": tc, "bytes": bc, "GBps": bc / (tc * 1e-3) / 1e9, "frac": bc / (tc * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
 
 
 def calibration_copy(nbytes, reps):

@@ -149,7 +149,9 @@ int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
 /* Kernel-level timing for the roofline: runs `reps` back-to-back launches of
    the propagator axis pass `axis` (0 x, 1 y, 2 z; 3, 4, 10, 12, 14 the
    xy-plane kernel forms 1, 2, 3, 4, 5; 5, 6, 7 the step-boundary x, y, z
-   kernels; 8, 9, 11, 13, 15 the step-boundary forms of 3, 4, 10, 12, 14)
+   kernels; 8, 9, 11, 13, 15 the step-boundary forms of 3, 4, 10, 12, 14;
+   16 the contour kernel over the current slice store, into scratch, which
+   needs the full store: SCFT_E_INVALID in checkpoint mode)
    on a copy of slice 0
    between two HIP events and returns the mean duration in ms.  Leaves the
    solver state unchanged except the scratch field. */

@@ -262,7 +262,7 @@ template <typename T, int P> struct alignas(P * sizeof(typename cplx<T>::type)) 
 // (one 16-byte load per slice: P = 2 in FP32, 1 in FP64; N % P == 0), 4 slice
 // pairs in flight.  Each point's sum runs in the order of the reference's
 // s-loop restricted to the pairs, the same for every P.
-template <typename T, int P>
+template <typename T, int P, int K = 4>
 __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__restrict__ slots, long N, int batch,
                                                 const int2 *__restrict__ nsab, const double *__restrict__ ffl,
                                                 double *__restrict__ phiA, double *__restrict__ phiB)
@@ -279,15 +279,15 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
     for (int p = 0; p < P; ++p) accA[p] = accB[p] = 0.0;
     const int half = Ns / 2;  // pairs s = 0..half (s == Ns - s only when Ns is even and s == half)
     int s = 0;
-    for (; s + 3 <= half; s += 4) {
-        G zs[4], zu[4];
+    for (; s + K - 1 <= half; s += K) {
+        G zs[K], zu[K];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
             zs[k] = zp[(long)(s + k) * BNg];
             zu[k] = zp[(long)(Ns - s - k) * BNg];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
             const int s1 = s + k, u1 = Ns - s - k;
             double wa1, wb1, wa2, wb2;
             trap_w(s1, NsA, Ns, wa1, wb1);
@@ -983,7 +983,8 @@ static constexpr int PLANE_FORMS = 5;  // xy-plane forms 1..5 (0: separate x and
 // pass-timing codes (scft_time_pass): 0..2 the axis passes, 5..7 their
 // step-boundary kernels; xy-plane form f has a pass code and a boundary code
 static constexpr int kPlaneCode[PLANE_FORMS + 1][2] = {{-1, -1}, {3, 8}, {4, 9}, {10, 11}, {12, 13}, {14, 15}};
-static constexpr int PASS_CODES = 16;
+static constexpr int CONTOUR_CODE = 16;  // the contour kernel over the full slice store
+static constexpr int PASS_CODES = 17;
 static void code_form(int code, int *form, bool *fused)
 {
     *form = 0;
@@ -1598,6 +1599,27 @@ static hipError_t enqueue_contour_segments(scft_ctx *c, double *phiA, double *ph
     return hipSuccess;
 }
 
+// The trapezoid contour over the full slice store (k_contour): FP32 slices
+// two points per thread (one 16-byte load per slice), FP64 one; 4 pairs of
+// slices in flight per thread.  (Measured at 64^3 and 128^3: 4 points per
+// thread or 2 / 8 pairs in flight move the kernel by -6 % to +25 %, under 1 %
+// of an iteration either way.)
+template <typename T>
+static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB)
+{
+    using C = typename cplx<T>::type;
+    if (sizeof(C) == 8 && c->N % 2 == 0) {
+        const dim3 gc((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
+        hipLaunchKernelGGL((k_contour<T, 2>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+    } else {
+        const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+        hipLaunchKernelGGL((k_contour<T, 1>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+    }
+    return hipGetLastError();
+}
+
 // getConc on the device for fields (fA, fB); mean_part: partial sums of fA to
 // subtract first (or null: fields used as given)
 template <typename T>
@@ -1617,18 +1639,7 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
                        c->ckpt ? (const int2 *)nullptr : (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
                        c->ffl);
-    if (!c->ckpt) {
-        if (sizeof(C) == 8 && c->N % 2 == 0) {
-            const dim3 gc((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
-            hipLaunchKernelGGL((k_contour<T, 2>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                               (const int2 *)c->nsab, c->ffl, phiA, phiB);
-        } else {
-            const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
-            hipLaunchKernelGGL((k_contour<T, 1>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                               (const int2 *)c->nsab, c->ffl, phiA, phiB);
-        }
-        return hipGetLastError();
-    }
+    if (!c->ckpt) return launch_contour<T>(c, phiA, phiB);
     return enqueue_contour_segments<T>(c, phiA, phiB);
 }
 
@@ -1951,13 +1962,29 @@ static int time_march(scft_ctx *ctx, int reps, float *ms)
     auto march = [&]() {
         return ctx->fp32 ? enqueue_march<float>(ctx, ctx->Ns_max, -1) : enqueue_march<double>(ctx, ctx->Ns_max, -1);
     };
-    HIPX(ctx, march());  // untimed: code loading, first touch
-    HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (int r = 0; r < reps; ++r) HIPX(ctx, march());
-    HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    HIPX(ctx, hipEventSynchronize(ctx->ev1));
+    // the march as the iterations run it: one captured graph, replayed (an
+    // eager march at small grids would time the host's launch rate)
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPX(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t em = march();
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (em != hipSuccess || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_err(ctx, SCFT_E_HIP, std::string("march capture: ") +
+                                            hipGetErrorString(em != hipSuccess ? em : ec));
+    }
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, ctx->stream);  // untimed: code loading, first touch
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(ge, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     float t = 0.0f;
-    HIPX(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    HIPX(ctx, e);
     *ms = t / reps;
     return SCFT_OK;
 }
@@ -1988,9 +2015,12 @@ static int time_pass_variant(scft_ctx *ctx, int axis, int reps, int variant, flo
     return SCFT_OK;
 }
 
+static int time_contour(scft_ctx *ctx, int reps, float *ms);
+
 extern "C" int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms)
 {
     if (!ctx || !ms || reps < 1 || axis < 0 || axis >= PASS_CODES) return SCFT_E_INVALID;
+    if (axis == CONTOUR_CODE) return time_contour(ctx, reps, ms);
     int form = 0;
     bool fused = false;
     code_form(axis, &form, &fused);
@@ -2044,7 +2074,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
             if (form > 0 && !plane_fits(ctx, ctx->n[0], ctx->n[1], form)) continue;
             ctx->use_plane = form;
             float t = 0.0f;
-            int rc = time_march(ctx, 2, &t);
+            int rc = time_march(ctx, 3, &t);
             if (rc) return rc;
             if (form == 0 || t < best) {
                 best = t;
@@ -2057,7 +2087,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
     // their axis's variant, and a pass that wins alone need not win there
     if (ctx->p.fuse_steps) {
         float best = 0.0f;
-        int rc = time_march(ctx, 2, &best);
+        int rc = time_march(ctx, 3, &best);
         if (rc) return rc;
         for (int a = plane ? 2 : 0; a < 3; ++a) {
             if (!ctx->active[a]) continue;
@@ -2071,7 +2101,7 @@ extern "C" int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms_out
                 }
                 const int prev = ctx->pass_variant[a];
                 ctx->pass_variant[a] = v;
-                if ((rc = time_march(ctx, 2, &t))) return rc;
+                if ((rc = time_march(ctx, 3, &t))) return rc;
                 if (t < best) best = t;
                 else ctx->pass_variant[a] = prev;
             }
@@ -2152,6 +2182,29 @@ static int ensure_adjoint_scratch(scft_ctx *c)
     const size_t BN = (size_t)c->batch * c->N;
     int rc = dalloc(c, (void **)&c->adj, 8 * BN * sizeof(double) + 2 * sizeof(double) * c->batch);
     return rc;
+}
+
+// `reps` contour launches over the current slice store into the adjoint
+// scratch (phi and Q of the context are not touched)
+static int time_contour(scft_ctx *ctx, int reps, float *ms)
+{
+    if (ctx->ckpt)
+        return set_err(ctx, SCFT_E_INVALID, "checkpoint mode sums the contour by segments (no single contour kernel)");
+    HIPX(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_adjoint_scratch(ctx);
+    if (rc) return rc;
+    const size_t BN = (size_t)ctx->batch * ctx->N;
+    double *pA = ctx->adj + 6 * BN, *pB = pA + BN;
+    auto one = [&]() { return ctx->fp32 ? launch_contour<float>(ctx, pA, pB) : launch_contour<double>(ctx, pA, pB); };
+    HIPX(ctx, one());  // untimed
+    HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) HIPX(ctx, one());
+    HIPX(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPX(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    HIPX(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t / reps;
+    return SCFT_OK;
 }
 
 template <typename T>

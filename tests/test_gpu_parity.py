@@ -389,3 +389,29 @@ def test_fused_march_matches_unfused(pkg, orc, shape, plane, prec, ckpt):
         pa, pb, Qo = orc.conc(op, orc.ktable(op), wA[1].copy(), wB[1].copy())
         assert abs(res[1][0][1] - Qo) / Qo < 1e-12
         assert _rel(res[1][1][1], pa) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_time_pass_contour_leaves_state(pkg, prec):
+    """scft_time_pass code 16 times the contour kernel into scratch: phi and Q
+    are unchanged afterwards, and checkpoint mode (no single contour kernel)
+    rejects the code."""
+    p = _params(pkg, (16, 16, 12), 4.0, 15.0, 0.4, 0.02, prec=prec)
+    wA, wB = pkg.fields.seeded_fields(16, 16, 12, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.1, 91)
+    fes = []
+    for timed in (False, True):
+        with pkg.Solver(p) as s:
+            s.set_fields(wA, wB)
+            s.concentrations()
+            before = [a.copy() for a in s.get_fields()]
+            if timed:
+                assert s.time_pass(16, 3) > 0
+            for a, b in zip(before, s.get_fields()):
+                assert np.array_equal(a, b)
+            s.update_fields()  # uses phi and Q of the concentrations call
+            fes.append(s.free_energy().copy())
+    assert np.array_equal(fes[0], fes[1])
+    with pkg.Solver(p.replace(checkpoint=1)) as s:
+        s.set_fields(wA, wB)
+        with pytest.raises(pkg.ScftError):
+            s.time_pass(16, 1)
