@@ -411,25 +411,28 @@ def main():
         forms["c3_128"]["fp32_ckpt_iters_per_s"] = 1e3 / mc
         tc.close()
     if not a.no_sweep:
-        log("[bench] c4 sweep: 256 instances, grids 32/48/64, FP64")
         inst = pkg.fields.sweep_instances(count=256, seed=4)
-        base = pkg.default_params(ds=0.01, precision=pkg.FP64)
-        sw = pkg.Sweep(base, inst)
-        sw.autotune(5)
-        for i, (wa, wb) in enumerate(pkg.fields.sweep_fields(inst, noise=0.1, seed=400)):
-            sw.set_fields(i, wa, wb)
-        sw.iterate(1)
-        ms = sw.time_iterations(3) / 3
         pts = sum(n ** 3 for (n, L, c, f) in inst)
-        nsum = sum(s_[1] + s_[2] for s_ in sw.sizes)
-        forms["c4_sweep"] = {
-            "instances": len(inst), "grids": {str(g): sum(1 for x in inst if x[0] == g) for g in (32, 48, 64)},
-            "ms_per_sweep_iteration": ms, "instance_iters_per_s": len(inst) * 1e3 / ms,
-            "grid_point_steps_per_s": pts * (nsum / len(inst)) * 1e3 / ms,
-            "iter_hbm_frac": sum(iter_bytes(n ** 3, s_[1] + s_[2], 8) for (n, L, c, f), s_ in zip(inst, sw.sizes))
-            * 1e3 / ms / 1e9 / HBM_PEAK_GBPS,
-        }
-        sw.close()
+        forms["c4_sweep"] = {"instances": len(inst),
+                             "grids": {str(g): sum(1 for x in inst if x[0] == g) for g in (32, 48, 64)}}
+        for prec, key in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32"), (pkg.FP32X, "fp32x")):
+            log(f"[bench] c4 sweep: 256 instances, grids 32/48/64, {key}")
+            base = pkg.default_params(ds=0.01, precision=prec)
+            sw = pkg.Sweep(base, inst)
+            sw.autotune(5)
+            for i, (wa, wb) in enumerate(pkg.fields.sweep_fields(inst, noise=0.1, seed=400)):
+                sw.set_fields(i, wa, wb)
+            sw.iterate(1)
+            ms = sw.time_iterations(3) / 3
+            nsum = sum(s_[1] + s_[2] for s_ in sw.sizes)
+            b = 8 if prec == pkg.FP64 else 4
+            forms["c4_sweep"][key] = {
+                "ms_per_sweep_iteration": ms, "instance_iters_per_s": len(inst) * 1e3 / ms,
+                "grid_point_steps_per_s": pts * (nsum / len(inst)) * 1e3 / ms,
+                "iter_hbm_frac": sum(iter_bytes(n ** 3, s_[1] + s_[2], b) for (n, L, c, f), s_ in zip(inst, sw.sizes))
+                * 1e3 / ms / 1e9 / HBM_PEAK_GBPS,
+            }
+            sw.close()
     if not a.no_adjoint:
         log("[bench] c5 adjoint: 64^3, 50 unrolled Anderson iterations, sqrt(Ns) checkpointing")
         forms["c5_adjoint"] = c5_adjoint(pkg)
