@@ -562,8 +562,9 @@ template <int MAXM>
 __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA, const double *__restrict__ wB,
                                                   const double *__restrict__ phiA, const double *__restrict__ phiB,
                                                   long N, int batch, int nblk, const double *__restrict__ chiN,
-                                                  double wcmp, double *__restrict__ wh, double *__restrict__ dh,
-                                                  const long *cnt, int m, double *__restrict__ part)
+                                                  double wcmp, double mix, double *__restrict__ wh,
+                                                  double *__restrict__ dh, const long *cnt, int m,
+                                                  double *__restrict__ part)
 {
     __shared__ double sh[(MAXM + 2) * RT / 64 + MAXM + 2 + 2];
     const int b = blockIdx.y;
@@ -582,8 +583,11 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
         const double psum = 1.0 - pa - pb;
         const double dA = hAB * pb + eta - a - wcmp * psum;
         const double dB = hAB * pa + eta - c - wcmp * psum;
-        wc[r] = a;
-        wc[N + r] = c;
+        // the field ring holds g = w + mix * d, all the update needs of the
+        // pair (w, d): w_new = g_0 + sum_k c_k (g_k - g_0), equal to
+        // (w_0 + sum c_k (w_k - w_0)) + mix * (d_0 + sum c_k (d_k - d_0))
+        wc[r] = a + mix * dA;
+        wc[N + r] = c + mix * dB;
         dc[r] = dA;
         dc[N + r] = dB;
 #pragma unroll
@@ -686,10 +690,9 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
 // w+ from the ring, then F partials with the updated fields
 __global__ void __launch_bounds__(RT) k_and_update(double *__restrict__ wA, double *__restrict__ wB,
                                                    const double *__restrict__ phiA, const double *__restrict__ phiB,
-                                                   long N, int batch, int nblk, int m, const long *cnt, double mix,
-                                                   const double *__restrict__ chiN, const double *__restrict__ wh,
-                                                   const double *__restrict__ dh, const double *__restrict__ coef,
-                                                   double *__restrict__ fpart)
+                                                   long N, int batch, int nblk, int m, const long *cnt,
+                                                   const double *__restrict__ chiN, const double *__restrict__ gh,
+                                                   const double *__restrict__ coef, double *__restrict__ fpart)
 {
     __shared__ double sh[2 * RT / 64 + 2];
     __shared__ double cs[AND_MAX];
@@ -700,22 +703,17 @@ __global__ void __launch_bounds__(RT) k_and_update(double *__restrict__ wA, doub
     if (threadIdx.x < AND_MAX) cs[threadIdx.x] = coef[b * AND_MAX + threadIdx.x];
     __syncthreads();
     const long slot = 2 * (long)batch * N;
-    const double *w0 = wh + rg.s[0] * slot + b * 2 * N;
-    const double *d0 = dh + rg.s[0] * slot + b * 2 * N;
+    const double *g0 = gh + rg.s[0] * slot + b * 2 * N;  // g = w + mix * d (k_and_resid)
     double fab = 0.0, fw = 0.0, inc = 0.0;
     for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
-        const double wa0 = w0[r], wb0 = w0[N + r], da0 = d0[r], db0 = d0[N + r];
-        double wa = wa0, wb = wb0, da = da0, db = db0;
+        const double ga0 = g0[r], gb0 = g0[N + r];
+        double a = ga0, cB = gb0;
         for (int k = 0; k < mp; ++k) {
-            const double *wk = wh + rg.s[k + 1] * slot + b * 2 * N;
-            const double *dk = dh + rg.s[k + 1] * slot + b * 2 * N;
+            const double *gk = gh + rg.s[k + 1] * slot + b * 2 * N;
             const double c = cs[k];
-            wa += c * (wk[r] - wa0);
-            wb += c * (wk[N + r] - wb0);
-            da += c * (dk[r] - da0);
-            db += c * (dk[N + r] - db0);
+            a += c * (gk[r] - ga0);
+            cB += c * (gk[N + r] - gb0);
         }
-        const double a = wa + mix * da, cB = wb + mix * db;
         const long i = b * N + r;
         wA[i] = a;
         wB[i] = cB;
@@ -786,7 +784,7 @@ struct scft_ctx {
     double *Q = nullptr, *fe = nullptr, *ffl = nullptr;
     void *tw[3] = {nullptr, nullptr, nullptr};
     void *kf[3] = {nullptr, nullptr, nullptr};
-    // Anderson
+    // Anderson: rings of g = w + mix * d (wh) and of the residuals d (dh), [m+1][batch][2][N]
     double *wh = nullptr, *dh = nullptr, *G = nullptr, *coef = nullptr, *aerr = nullptr, *apart = nullptr;
     long *and_cnt = nullptr;  // device: Anderson updates done since the last set_fields
     int and_maxm = 0;         // accumulator bucket (8, 16 or 32) >= anderson_m
@@ -1664,21 +1662,20 @@ static hipError_t enqueue_update(scft_ctx *c)
         const int m = c->p.anderson_m;
         if (c->and_maxm == 8)
             hipLaunchKernelGGL(k_and_resid<8>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
                                c->apart);
         else if (c->and_maxm == 16)
             hipLaunchKernelGGL(k_and_resid<16>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
                                c->apart);
         else
             hipLaunchKernelGGL(k_and_resid<32>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->wh, c->dh, c->and_cnt, m,
+                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
                                c->apart);
         hipLaunchKernelGGL(k_and_solve, dim3(c->batch), dim3(RT), 0, c->stream, c->apart, c->nblk, c->and_maxm, m,
                            c->and_cnt, c->G, c->coef, c->aerr, c->p.anderson_reg);
         hipLaunchKernelGGL(k_and_update, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch,
-                           c->nblk, m, c->and_cnt, c->p.anderson_mix, (const double *)c->chiN_b, c->wh, c->dh,
-                           c->coef, c->fpart);
+                           c->nblk, m, c->and_cnt, (const double *)c->chiN_b, c->wh, c->coef, c->fpart);
     }
     hipLaunchKernelGGL(k_fe_final, dim3(c->batch), dim3(RT), 0, c->stream, c->fpart, c->nblk, c->Q, c->N, c->fe,
                        c->p.mixing == SCFT_MIX_ANDERSON ? c->and_cnt : nullptr);
