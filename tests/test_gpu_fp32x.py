@@ -136,3 +136,25 @@ def test_fp32x_simple_mixing_gradient(pkg):
     dl = abs(res[FP32X][0][0] - res[FP64][0][0]) / abs(res[FP64][0][0])
     print(f"FP32X vs FP64, 20 simple-mixing iterations: rel dloss={dl:.2e} rel dgrad={dg:.2e}")
     assert dl < 1e-6 and dg < 1e-5
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 12, 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 128, 144,
+                               160, 192, 256])
+def test_fp32x_every_line_length(pkg, n):
+    """Every line length of the pass table in the FP32X kernels (FP64 registers
+    and LDS, so twice the LDS per line): one getConc on an (n, 6, 4) grid and
+    on a (4, 2, n) grid (n along x, then along z) against the FP64 form."""
+    print("case", n, flush=True)
+    for shape in ((n, 6, 4), (4, 2, n)):
+        out = {}
+        for prec in (FP64, FP32X):
+            p = _params(pkg, shape, 4.0, 15.0, 0.4, 0.02, prec)
+            wA, wB = pkg.fields.seeded_fields(*shape, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.2, 97)
+            with pkg.Solver(p) as s:
+                s.set_fields(wA, wB)
+                Q = s.concentrations()[0]
+                out[prec] = (Q, s.get_fields()[0][0].copy())
+        eQ = abs(out[FP32X][0] - out[FP64][0]) / out[FP64][0]
+        ephi = float(np.max(np.abs(out[FP32X][1] - out[FP64][1])))
+        print(f"  {shape}: relQ={eQ:.2e} max|dphiA|={ephi:.2e}")
+        assert eQ < 1e-6 and ephi < 1e-6, (shape, eQ, ephi)
