@@ -313,7 +313,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this solver runs one GPU per process and this build measures one GPU only; "
-              "independent SCFT instances shard across GPUs with no exchange (see DESIGN.md). "
+              "independent SCFT instances shard across GPUs with no exchange. "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()

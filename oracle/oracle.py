@@ -1,7 +1,8 @@
 """ctypes binding of the CPU FP64 oracle (oracle/libscft_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check in
-__graft_entry__.py and by bench.py's cpu_baseline leg, and only as the checker.
+__graft_entry__.py and by bench.py's cpu_baseline leg, and only to check results
+(and, in bench.py, as the timed CPU baseline).
 The product path (scft_ver1.01_amd) never imports this module.
 
 Arrays are numpy float64, C-contiguous, grid index r = (k*Ny + j)*Nx + i.

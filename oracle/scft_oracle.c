@@ -1,7 +1,7 @@
 /*
  * scft_oracle.c -- CPU FP64 restatement of the reference SCFT method.
  *
- * TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline.
+ * TEST INFRASTRUCTURE ONLY: the parity reference of the tests and the timed CPU baseline.
  * The product never links or calls this file.
  *
  * Function-by-function correspondence with the reference (ren5885a/SCFT_Ver1.01):

@@ -1,7 +1,7 @@
 /*
  * scft_oracle.h -- CPU FP64 restatement of the reference SCFT method.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the timed
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference and the timed
  * CPU baseline ("port").  Nothing in the product (libscft_hip.so, the driver,
  * the Python host package) links, loads or calls it; only tests/, the smoke
  * check and bench.py's cpu_baseline leg do.

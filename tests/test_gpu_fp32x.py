@@ -7,8 +7,8 @@ point to point, so Q's error averages over the grid: relQ * sqrt(N) ~ 2-10e-6
 (5e-8 at 32^3, 7e-9 at 64^3, 1.4e-9 at 128^3), where the FP32 form's FFT
 arithmetic leaves a systematic ~1e-7 at every size.  The pointwise phi error
 is the same storage floor in both forms (~1e-6 after Ns = 200 steps).  The
-bounds are written in each test, and the north-star bar |dF| < 1e-7 is
-checked at the full 64^3 and 128^3 sizes."""
+bounds are written in each test, and |dF| < 1e-7 against FP64 is checked
+at the full 64^3 and 128^3 sizes."""
 import numpy as np
 import pytest
 
@@ -68,7 +68,7 @@ def test_fp32x_c1_iterations_vs_oracle(pkg, orc):
 @pytest.mark.parametrize("name", ["c2_cylinder_64", "c3_gyroid_128"])
 def test_fp32x_full_size_single_eval(pkg, name):
     """getConc, update and F at the full 64^3 / 128^3, Ns = 200 workloads on
-    their seeded fields: |dF| < 1e-7 against the FP64 form (the north-star bar)."""
+    their seeded fields: |dF| < 1e-7 against the FP64 form."""
     print("case", name, flush=True)
     w = pkg.fields.WORKLOADS[name]
     wA, wB = pkg.fields.workload_fields(name)
