@@ -22,12 +22,20 @@
 // in-register FFT (pass_kernels.inc).
 #include <hip/hip_runtime.h>
 
+#include <sys/mman.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2354,12 +2362,151 @@ bool host_solve(int n, std::vector<double> A, std::vector<double> &x)
     return true;
 }
 
+// Host threads for the loop adjoint's passes over grid points: a fixed pool
+// (up to 16 threads) that runs job(c) for chunks c = 0..n-1, the calling
+// thread taking chunks too.  A call made while another host thread holds the
+// pool (contexts on several threads), or from a process forked after the
+// pool started (whose copy has no threads), runs its chunks serially.
+class HostPool {
+public:
+    static HostPool &get()
+    {
+        static HostPool pool;
+        return pool;
+    }
+    void run(long nchunks, const std::function<void(long)> &job)
+    {
+        std::unique_lock<std::mutex> owner(run_m_, std::try_to_lock);
+        if (!owner.owns_lock() || workers_.empty() || nchunks <= 1 || getpid() != pid_) {
+            for (long c = 0; c < nchunks; ++c) job(c);
+            return;
+        }
+        std::unique_lock<std::mutex> lk(m_);
+        job_ = &job;
+        n_ = nchunks;
+        next_.store(0);
+        busy_ = (int)workers_.size();
+        ++gen_;
+        lk.unlock();
+        cv_.notify_all();
+        for (long c; (c = next_.fetch_add(1)) < nchunks;) job(c);
+        lk.lock();
+        done_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    HostPool() : pid_(getpid())
+    {
+        const unsigned hc = std::thread::hardware_concurrency();
+        const unsigned t = std::min(16u, hc ? hc : 1u);
+        for (unsigned i = 1; i < t; ++i) workers_.emplace_back([this] { work(); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_) w.join();
+    }
+    void work()
+    {
+        unsigned long seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            const std::function<void(long)> *job = job_;
+            const long n = n_;
+            lk.unlock();
+            for (long c; (c = next_.fetch_add(1)) < n;) (*job)(c);
+            lk.lock();
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex run_m_;  // held by the thread whose job the pool runs
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(long)> *job_ = nullptr;
+    std::atomic<long> next_{0};
+    long n_ = 0;
+    int busy_ = 0;
+    unsigned long gen_ = 0;
+    bool stop_ = false;
+    pid_t pid_;
+};
+
+// [0, n) in fixed chunks of HOST_CHUNK points: f(lo, hi) per chunk.  Sums
+// combine per-chunk partials in chunk order, so results do not depend on the
+// number of threads.
+constexpr long HOST_CHUNK = 16384;
+template <typename F> void host_for(long n, F &&f)
+{
+    const long nc = (n + HOST_CHUNK - 1) / HOST_CHUNK;
+    HostPool::get().run(nc, [&](long c) { f(c * HOST_CHUNK, std::min(n, (c + 1) * HOST_CHUNK)); });
+}
+// f(lo, hi, acc) adds its chunk's terms to acc[0..width); returns the totals
+template <typename F> std::vector<double> host_sum(long n, int width, F &&f)
+{
+    const long nc = (n + HOST_CHUNK - 1) / HOST_CHUNK;
+    std::vector<double> part((size_t)nc * width, 0.0);
+    HostPool::get().run(nc, [&](long c) {
+        f(c * HOST_CHUNK, std::min(n, (c + 1) * HOST_CHUNK), part.data() + (size_t)c * width);
+    });
+    std::vector<double> tot(width, 0.0);
+    for (long c = 0; c < nc; ++c)
+        for (int j = 0; j < width; ++j) tot[j] += part[(size_t)c * width + j];
+    return tot;
+}
+
+// K host slices of n doubles in one 2 MB-aligned block, advised for
+// transparent huge pages and zeroed by the pool up front, so that the loop
+// adjoint's passes do not take first-touch page faults (4 KB each) on the
+// trajectory they write.
+class HostSlices {
+public:
+    struct Ref {
+        double *p;
+        double *data() const { return p; }
+        double &operator[](size_t i) const { return p[i]; }
+    };
+    HostSlices(size_t k, size_t n) : n_(n)
+    {
+        const size_t align = (size_t)2 << 20;
+        bytes_ = ((k * n * sizeof(double)) + align - 1) / align * align;
+        base_ = bytes_ ? static_cast<double *>(std::aligned_alloc(align, bytes_)) : nullptr;
+        if (!base_) return;
+        (void)madvise(base_, bytes_, MADV_HUGEPAGE);
+        double *b = base_;
+        host_for((long)(bytes_ / sizeof(double)), [b](long lo, long hi) {
+            std::memset(b + lo, 0, sizeof(double) * (hi - lo));
+        });
+    }
+    ~HostSlices() { std::free(base_); }
+    HostSlices(const HostSlices &) = delete;
+    HostSlices &operator=(const HostSlices &) = delete;
+    bool ok() const { return base_ != nullptr; }
+    Ref operator[](size_t k) const { return Ref{base_ + k * n_}; }
+
+private:
+    double *base_ = nullptr;
+    size_t n_ = 0, bytes_ = 0;
+};
+
 void center(double *w, long N)
 {
-    double s = 0.0;
-    for (long r = 0; r < N; ++r) s += w[r];
+    const double s = host_sum(N, 1, [w](long lo, long hi, double *a) {
+        for (long r = lo; r < hi; ++r) a[0] += w[r];
+    })[0];
     const double m = s / (double)N;
-    for (long r = 0; r < N; ++r) w[r] -= m;
+    host_for(N, [w, m](long lo, long hi) {
+        for (long r = lo; r < hi; ++r) w[r] -= m;
+    });
 }
 
 }  // namespace
@@ -2384,26 +2531,47 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     const double lam = anderson ? ctx->p.anderson_mix : ctx->p.wopt;
     const int K = n_iter;
     // per-iteration state (FP64): centered fields, phi, residual
-    std::vector<std::vector<double>> wt(K, std::vector<double>(2 * BN)), phi(K, std::vector<double>(2 * BN)),
-        dd(K, std::vector<double>(2 * BN));
+    HostSlices wt(K, 2 * BN), phi(K, 2 * BN), dd(K, 2 * BN);
+    if (!wt.ok() || !phi.ok() || !dd.ok())
+        return set_err(ctx, SCFT_E_NOMEM, "scft_adjoint_loop: host trajectory allocation failed");
     // Anderson per update k, per instance: mp, coefficients, U, solve status
     std::vector<int> mpk(K, 0);
     std::vector<std::vector<double>> ck(K), Uk(K);
     std::vector<std::vector<char>> okk(K);
     double *dev = ctx->adj;
-    auto inst = [&](std::vector<double> &v, int f, int b) { return v.data() + (size_t)f * BN + (size_t)b * N; };
-    auto run_conc = [&](const std::vector<double> &w, std::vector<double> &ph) -> int {
-        std::vector<double> u(2 * BN);
-        for (size_t i = 0; i < BN; ++i) {
-            u[i] = w[i] + h[i];
-            u[BN + i] = w[BN + i];
+    auto inst = [&](auto &&v, int f, int b) { return v.data() + (size_t)f * BN + (size_t)b * N; };
+    // pinned staging for the per-iteration transfers ([2][BN] each), held for this call
+    struct Pinned {
+        double *p = nullptr;
+        ~Pinned()
+        {
+            if (p) (void)hipHostFree(p);
         }
-        HIPX(ctx, hipMemcpyAsync(dev, u.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+    } st0, st1;
+    HIPX(ctx, hipHostMalloc((void **)&st0.p, 16 * BN, hipHostMallocDefault));
+    HIPX(ctx, hipHostMalloc((void **)&st1.p, 16 * BN, hipHostMallocDefault));
+    auto copy_par = [](double *dst, const double *src, size_t n) {
+        host_for((long)n, [&](long lo, long hi) { std::memcpy(dst + lo, src + lo, sizeof(double) * (hi - lo)); });
+    };
+    // u = (w_A + h, w_B) into st0
+    auto stage_u = [&](const double *wp) {
+        double *u = st0.p;
+        host_for((long)BN, [&](long lo, long hi) {
+            for (long i = lo; i < hi; ++i) {
+                u[i] = wp[i] + h[i];
+                u[BN + i] = wp[BN + i];
+            }
+        });
+    };
+    auto run_conc = [&](const double *w, double *ph) -> int {
+        stage_u(w);
+        HIPX(ctx, hipMemcpyAsync(dev, st0.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
         HIPX(ctx, enqueue_conc_scratch(ctx));
-        HIPX(ctx, hipMemcpyAsync(ph.data(), dev + 6 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(ctx, hipMemcpyAsync(st1.p, dev + 6 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
         std::vector<double> q(nb);
         HIPX(ctx, hipMemcpyAsync(q.data(), ctx->Q, 8 * nb, hipMemcpyDeviceToHost, ctx->stream));
         HIPX(ctx, hipStreamSynchronize(ctx->stream));
+        copy_par(ph, st1.p, 2 * BN);
         for (int b = 0; b < nb; ++b)
             if (!std::isfinite(q[b]) || !(q[b] > 0.0))
                 return set_err(ctx, SCFT_E_NONFINITE, "Q not finite and > 0 in the adjoint's forward loop");
@@ -2414,16 +2582,22 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     std::memcpy(w.data(), w0A, 8 * BN);
     std::memcpy(w.data() + BN, w0B, 8 * BN);
     for (int k = 0; k < K; ++k) {
-        wt[k] = w;
+        copy_par(wt[k].data(), w.data(), 2 * BN);
         for (int b = 0; b < nb; ++b)
             for (int f = 0; f < 2; ++f) center(inst(wt[k], f, b), N);
-        if ((rc = run_conc(wt[k], phi[k]))) return rc;
+        if ((rc = run_conc(wt[k].data(), phi[k].data()))) return rc;
         if (k == K - 1) break;
-        for (size_t i = 0; i < BN; ++i) {
-            const double a = wt[k][i], c = wt[k][BN + i], pa = phi[k][i], pb = phi[k][BN + i];
-            const double eta = (a + c - chiN) / 2, psum = 1.0 - pa - pb;
-            dd[k][i] = chiN * pb + eta - a - wcmp * psum;
-            dd[k][BN + i] = chiN * pa + eta - c - wcmp * psum;
+        {
+            const double *wk = wt[k].data(), *pk = phi[k].data();
+            double *dk = dd[k].data();
+            host_for((long)BN, [&](long lo, long hi) {
+                for (long i = lo; i < hi; ++i) {
+                    const double a = wk[i], c = wk[BN + i], pa = pk[i], pb = pk[BN + i];
+                    const double eta = (a + c - chiN) / 2, psum = 1.0 - pa - pb;
+                    dk[i] = chiN * pb + eta - a - wcmp * psum;
+                    dk[BN + i] = chiN * pa + eta - c - wcmp * psum;
+                }
+            });
         }
         const int mp = anderson ? std::min(k, m) : 0;
         mpk[k] = mp;
@@ -2435,17 +2609,25 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
             if (mp > 0) {
                 // U_ij = <d_k - d_{k-i}, d_k - d_{k-j}>, V_i = <d_k - d_{k-i}, d_k>
                 // one pass over the points accumulates every <Delta_i, Delta_j> and <Delta_i, d_k>
-                std::vector<double> U(mp * mp, 0.0), V(mp, 0.0), Dl(mp);
+                std::vector<double> U(mp * mp, 0.0), V(mp, 0.0);
                 for (int f = 0; f < 2; ++f) {
                     const double *d0 = inst(dd[k], f, b);
                     std::vector<const double *> dk(mp + 1);
                     for (int i = 1; i <= mp; ++i) dk[i] = inst(dd[k - i], f, b);
-                    for (long r = 0; r < N; ++r) {
-                        for (int i = 0; i < mp; ++i) Dl[i] = d0[r] - dk[i + 1][r];
-                        for (int i = 0; i < mp; ++i) {
-                            V[i] += Dl[i] * d0[r];
-                            for (int j = i; j < mp; ++j) U[i * mp + j] += Dl[i] * Dl[j];
+                    // per chunk: acc[0..mp) = V, acc[mp + i*mp + j] = U_ij (j >= i)
+                    const std::vector<double> t = host_sum(N, mp + mp * mp, [&](long lo, long hi, double *acc) {
+                        double Dl[AND_MAX];
+                        for (long r = lo; r < hi; ++r) {
+                            for (int i = 0; i < mp; ++i) Dl[i] = d0[r] - dk[i + 1][r];
+                            for (int i = 0; i < mp; ++i) {
+                                acc[i] += Dl[i] * d0[r];
+                                for (int j = i; j < mp; ++j) acc[mp + i * mp + j] += Dl[i] * Dl[j];
+                            }
                         }
+                    });
+                    for (int i = 0; i < mp; ++i) {
+                        V[i] += t[i];
+                        for (int j = i; j < mp; ++j) U[i * mp + j] += t[mp + i * mp + j];
                     }
                 }
                 for (int i = 0; i < mp; ++i)
@@ -2465,35 +2647,48 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
             for (int f = 0; f < 2; ++f) {
                 const double *w0 = inst(wt[k], f, b), *d0 = inst(dd[k], f, b);
                 double *wo = w.data() + (size_t)f * BN + (size_t)b * N;
-                for (long r = 0; r < N; ++r) {
-                    double wn = w0[r], dn = d0[r];
-                    for (int i = 1; i <= mp; ++i) {
-                        wn += c[i - 1] * (inst(wt[k - i], f, b)[r] - w0[r]);
-                        dn += c[i - 1] * (inst(dd[k - i], f, b)[r] - d0[r]);
-                    }
-                    wo[r] = wn + lam * dn;
+                std::vector<const double *> wi(mp + 1), di(mp + 1);
+                for (int i = 1; i <= mp; ++i) {
+                    wi[i] = inst(wt[k - i], f, b);
+                    di[i] = inst(dd[k - i], f, b);
                 }
+                host_for(N, [&](long lo, long hi) {
+                    for (long r = lo; r < hi; ++r) {
+                        double wn = w0[r], dn = d0[r];
+                        for (int i = 1; i <= mp; ++i) {
+                            wn += c[i - 1] * (wi[i][r] - w0[r]);
+                            dn += c[i - 1] * (di[i][r] - d0[r]);
+                        }
+                        wo[r] = wn + lam * dn;
+                    }
+                });
             }
         }
     }
     for (int b = 0; b < nb; ++b) {
-        double L = 0.0;
-        const double *pa = inst(phi[K - 1], 0, b);
-        for (long r = 0; r < N; ++r) {
-            const double e = pa[r] - target[(size_t)b * N + r];
-            L += e * e;
-        }
+        const double *pa = inst(phi[K - 1], 0, b), *tg = target + (size_t)b * N;
+        const double L = host_sum(N, 1, [&](long lo, long hi, double *acc) {
+            for (long r = lo; r < hi; ++r) {
+                const double e = pa[r] - tg[r];
+                acc[0] += e * e;
+            }
+        })[0];
         if (loss) loss[b] = L;
     }
     if (phiA_last) std::memcpy(phiA_last, phi[K - 1].data(), 8 * BN);
     if (!grad_h) return SCFT_OK;
     // ---- reverse ----
-    std::vector<std::vector<double>> Wtb(K, std::vector<double>(2 * BN, 0.0)), Db(K, std::vector<double>(2 * BN, 0.0));
+    HostSlices Wtb(K, 2 * BN), Db(K, 2 * BN);  // zeroed
+    if (!Wtb.ok() || !Db.ok())
+        return set_err(ctx, SCFT_E_NOMEM, "scft_adjoint_loop: host adjoint allocation failed");
     std::vector<double> wbar(2 * BN, 0.0), hbar(BN, 0.0), pbar(2 * BN), ubar(2 * BN);
     for (int k = K - 1; k >= 0; --k) {
         std::fill(pbar.begin(), pbar.end(), 0.0);
         if (k == K - 1) {
-            for (size_t i = 0; i < BN; ++i) pbar[i] = 2.0 * (phi[k][i] - target[i]);
+            const double *pk = phi[k].data();
+            host_for((long)BN, [&](long lo, long hi) {
+                for (long i = lo; i < hi; ++i) pbar[i] = 2.0 * (pk[i] - target[i]);
+            });
         } else {
             const int mp = mpk[k];
             for (int b = 0; b < nb; ++b) {
@@ -2503,30 +2698,39 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                 for (int f = 0; f < 2; ++f) {
                     const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
                     double *wt0 = inst(Wtb[k], f, b), *d0 = inst(Db[k], f, b);
-                    for (long r = 0; r < N; ++r) {
-                        wt0[r] += (1.0 - sc) * wb[r];
-                        d0[r] += lam * (1.0 - sc) * wb[r];
-                    }
+                    std::vector<double *> wti(mp + 1), di(mp + 1);
                     for (int i = 1; i <= mp; ++i) {
-                        double *wti = inst(Wtb[k - i], f, b), *di = inst(Db[k - i], f, b);
-                        for (long r = 0; r < N; ++r) {
-                            wti[r] += c[i - 1] * wb[r];
-                            di[r] += lam * c[i - 1] * wb[r];
-                        }
+                        wti[i] = inst(Wtb[k - i], f, b);
+                        di[i] = inst(Db[k - i], f, b);
                     }
+                    host_for(N, [&](long lo, long hi) {
+                        for (long r = lo; r < hi; ++r) {
+                            wt0[r] += (1.0 - sc) * wb[r];
+                            d0[r] += lam * (1.0 - sc) * wb[r];
+                            for (int i = 1; i <= mp; ++i) {
+                                wti[i][r] += c[i - 1] * wb[r];
+                                di[i][r] += lam * c[i - 1] * wb[r];
+                            }
+                        }
+                    });
                 }
                 if (mp > 0 && okk[k][b]) {
                     // cbar_i = <wbar, (wt_{k-i} - wt_k) + lam (d_{k-i} - d_k)>
                     std::vector<double> cbar(mp, 0.0);
-                    for (int i = 1; i <= mp; ++i) {
-                        double v = 0.0;
-                        for (int f = 0; f < 2; ++f) {
-                            const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
-                            const double *w0 = inst(wt[k], f, b), *wi = inst(wt[k - i], f, b);
-                            const double *d0 = inst(dd[k], f, b), *di = inst(dd[k - i], f, b);
-                            for (long r = 0; r < N; ++r) v += wb[r] * ((wi[r] - w0[r]) + lam * (di[r] - d0[r]));
+                    for (int f = 0; f < 2; ++f) {
+                        const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
+                        const double *w0 = inst(wt[k], f, b), *d0 = inst(dd[k], f, b);
+                        std::vector<const double *> wi(mp + 1), di(mp + 1);
+                        for (int i = 1; i <= mp; ++i) {
+                            wi[i] = inst(wt[k - i], f, b);
+                            di[i] = inst(dd[k - i], f, b);
                         }
-                        cbar[i - 1] = v;
+                        const std::vector<double> t = host_sum(N, mp, [&](long lo, long hi, double *acc) {
+                            for (long r = lo; r < hi; ++r)
+                                for (int i = 1; i <= mp; ++i)
+                                    acc[i - 1] += wb[r] * ((wi[i][r] - w0[r]) + lam * (di[i][r] - d0[r]));
+                        });
+                        for (int i = 0; i < mp; ++i) cbar[i] += t[i];
                     }
                     std::vector<double> U(Uk[k].begin() + (size_t)b * mp * mp,
                                           Uk[k].begin() + (size_t)(b + 1) * mp * mp);
@@ -2548,54 +2752,66 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                             dk[i] = inst(dd[k - i], f, b);
                             dbk[i] = inst(Db[k - i], f, b);
                         }
-                        for (long r = 0; r < N; ++r) {
-                            double S1 = 0.0, S2 = 0.0;
-                            for (int j = 1; j <= mp; ++j) {
-                                const double Dj = d0[r] - dk[j][r];
-                                S1 += c[j - 1] * Dj;
-                                S2 += Vb[j - 1] * Dj;
+                        host_for(N, [&](long lo, long hi) {
+                            for (long r = lo; r < hi; ++r) {
+                                double S1 = 0.0, S2 = 0.0;
+                                for (int j = 1; j <= mp; ++j) {
+                                    const double Dj = d0[r] - dk[j][r];
+                                    S1 += c[j - 1] * Dj;
+                                    S2 += Vb[j - 1] * Dj;
+                                }
+                                double acc0 = 0.0;
+                                for (int i = 1; i <= mp; ++i) {
+                                    const double Di = d0[r] - dk[i][r];
+                                    const double Dbi = Vb[i - 1] * (d0[r] - S1) - c[i - 1] * S2 + 2.0 * rdiag * Di;
+                                    acc0 += Vb[i - 1] * Di + Dbi;
+                                    dbk[i][r] -= Dbi;
+                                }
+                                db0[r] += acc0;
                             }
-                            double acc0 = 0.0;
-                            for (int i = 1; i <= mp; ++i) {
-                                const double Di = d0[r] - dk[i][r];
-                                const double Dbi = Vb[i - 1] * (d0[r] - S1) - c[i - 1] * S2 + 2.0 * rdiag * Di;
-                                acc0 += Vb[i - 1] * Di + Dbi;
-                                dbk[i][r] -= Dbi;
-                            }
-                            db0[r] += acc0;
-                        }
+                        });
                     }
                 }
             }
             // residual d_k = R(wt_k, phi_k): linear adjoint
-            for (size_t i = 0; i < BN; ++i) {
-                const double da = Db[k][i], db = Db[k][BN + i];
-                Wtb[k][i] += -0.5 * da + 0.5 * db;
-                Wtb[k][BN + i] += 0.5 * da - 0.5 * db;
-                pbar[i] += wcmp * da + (chiN + wcmp) * db;
-                pbar[BN + i] += (chiN + wcmp) * da + wcmp * db;
+            {
+                const double *dbk = Db[k].data();
+                double *wtb = Wtb[k].data();
+                host_for((long)BN, [&](long lo, long hi) {
+                    for (long i = lo; i < hi; ++i) {
+                        const double da = dbk[i], db = dbk[BN + i];
+                        wtb[i] += -0.5 * da + 0.5 * db;
+                        wtb[BN + i] += 0.5 * da - 0.5 * db;
+                        pbar[i] += wcmp * da + (chiN + wcmp) * db;
+                        pbar[BN + i] += (chiN + wcmp) * da + wcmp * db;
+                    }
+                });
             }
         }
         // chain adjoint on the device: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
         {
-            std::vector<double> u(2 * BN);
-            for (size_t i = 0; i < BN; ++i) {
-                u[i] = wt[k][i] + h[i];
-                u[BN + i] = wt[k][BN + i];
-            }
-            HIPX(ctx, hipMemcpyAsync(dev, u.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
-            HIPX(ctx, hipMemcpyAsync(dev + 2 * BN, pbar.data(), 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+            stage_u(wt[k].data());
+            copy_par(st1.p, pbar.data(), 2 * BN);
+            HIPX(ctx, hipMemcpyAsync(dev, st0.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
+            HIPX(ctx, hipMemcpyAsync(dev + 2 * BN, st1.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
             HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0) : enqueue_conc_adjoint<double>(ctx, 0.0));
-            HIPX(ctx, hipMemcpyAsync(ubar.data(), dev + 4 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
+            // into st0: its upload precedes this copy on the stream
+            HIPX(ctx, hipMemcpyAsync(st0.p, dev + 4 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
             HIPX(ctx, hipStreamSynchronize(ctx->stream));
+            copy_par(ubar.data(), st0.p, 2 * BN);
         }
-        for (size_t i = 0; i < BN; ++i) {
-            hbar[i] += ubar[i];
-            Wtb[k][i] += ubar[i];
-            Wtb[k][BN + i] += ubar[BN + i];
+        {
+            double *wtb = Wtb[k].data();
+            host_for((long)BN, [&](long lo, long hi) {
+                for (long i = lo; i < hi; ++i) {
+                    hbar[i] += ubar[i];
+                    wtb[i] += ubar[i];
+                    wtb[BN + i] += ubar[BN + i];
+                }
+            });
         }
         // mean removal P is symmetric: wbar(k-1) = P Wtbar(k)
-        wbar = Wtb[k];
+        copy_par(wbar.data(), Wtb[k].data(), 2 * BN);
         for (int b = 0; b < nb; ++b)
             for (int f = 0; f < 2; ++f) center(wbar.data() + (size_t)f * BN + (size_t)b * N, N);
     }
