@@ -175,11 +175,15 @@ int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *uB, const d
    reference counterpart).  External field h (host [batch][N]) acts on the A
    blocks: iteration k evaluates the chain at (P wA + h, P wB) (P removes the
    mean), and between evaluations updates w with the context's mixing (simple
-   or Anderson; FP64 on the host).  Starting from (w0A, w0B), n_iter chain
+   or Anderson, in FP64).  Starting from (w0A, w0B), n_iter chain
    evaluations are made; loss[b] = sum_r (phiA - target)^2 of the last one.
    grad_h (may be NULL: forward only) receives dloss/dh by the discrete
    adjoint of every step (mixing, least squares, chain, mean removal).
-   phiA_last (may be NULL) receives the last phiA. */
+   phiA_last (may be NULL) receives the last phiA.  The trajectory and its
+   adjoint stay in device memory: 2 * batch * N doubles per iteration for
+   each of 3 arrays (5 with grad_h), held by the context after the call and
+   grown when a later call needs more; only the m x m least-squares solves
+   run on the host. */
 int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const double *w0B, int n_iter,
                       const double *target, double *loss, double *grad_h, double *phiA_last);
 

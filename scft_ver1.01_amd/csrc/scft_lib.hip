@@ -22,20 +22,12 @@
 // in-register FFT (pass_kernels.inc).
 #include <hip/hip_runtime.h>
 
-#include <sys/mman.h>
-#include <unistd.h>
-
 #include <algorithm>
-#include <atomic>
 #include <cmath>
-#include <condition_variable>
 #include <cstdio>
 #include <cstring>
-#include <functional>
 #include <memory>
-#include <mutex>
 #include <string>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -768,6 +760,9 @@ struct scft_ctx {
     int ck_K = 1, ck_n = 1;  // checkpoint spacing and count
     // adjoint scratch (allocated on first use): uA, uB, lamA, lamB, gA, gB, phiA, phiB
     double *adj = nullptr;
+    // scft_adjoint_loop's trajectory and reduction buffers (grown on demand)
+    double *loop = nullptr;
+    size_t loop_bytes = 0;
     // per-instance chain parameters (equal entries for a uniform context)
     int2 *nsab = nullptr;      // device (NsA, NsB) per instance
     double *chiN_b = nullptr;  // device chiN per instance
@@ -2319,7 +2314,7 @@ extern "C" int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *
 
 // ---------------------------------------------------------------------------
 // Adjoint of the unrolled SCFT loop with an external field h on the A blocks.
-// Forward, k = 0..n-1 (FP64 host vectors, device chain evaluations):
+// Forward, k = 0..n-1:
 //   wt = P w (mean removed per instance and field), u = (wtA + h, wtB),
 //   (phi, Q) = conc(u); if k < n-1: d = R(wt, phi), w = mix(wt, d, history)
 // Loss = sum_r (phiA(n-1) - target)^2 per instance.  R is the residual that
@@ -2327,8 +2322,14 @@ extern "C" int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *
 // mix is simple mixing (w = wt + wopt d) or Anderson (the library's
 // algorithm, with U and V formed directly from the differences).  The reverse
 // sweep runs the Anderson least-squares adjoint (cbar -> Vbar = U^-1 cbar,
-// Ubar = -Vbar c^T), the residual's linear adjoint, the chain adjoint on the
-// device, and P (symmetric) for the mean removal.
+// Ubar = -Vbar c^T), the residual's linear adjoint, the chain adjoint, and P
+// (symmetric) for the mean removal.
+//
+// Everything runs on the device in FP64 (the chain in the context's
+// precision): the trajectory (wt, phi, d per iteration) and its adjoint
+// accumulators stay in device memory, the passes over grid points are the
+// kernels below, and only the m x m least-squares solves (forward and
+// reverse) run on the host, on sums the device reduces in a fixed order.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -2362,151 +2363,287 @@ bool host_solve(int n, std::vector<double> A, std::vector<double> &x)
     return true;
 }
 
-// Host threads for the loop adjoint's passes over grid points: a fixed pool
-// (up to 16 threads) that runs job(c) for chunks c = 0..n-1, the calling
-// thread taking chunks too.  A call made while another host thread holds the
-// pool (contexts on several threads), or from a process forked after the
-// pool started (whose copy has no threads), runs its chunks serially.
-class HostPool {
-public:
-    static HostPool &get()
-    {
-        static HostPool pool;
-        return pool;
-    }
-    void run(long nchunks, const std::function<void(long)> &job)
-    {
-        std::unique_lock<std::mutex> owner(run_m_, std::try_to_lock);
-        if (!owner.owns_lock() || workers_.empty() || nchunks <= 1 || getpid() != pid_) {
-            for (long c = 0; c < nchunks; ++c) job(c);
-            return;
-        }
-        std::unique_lock<std::mutex> lk(m_);
-        job_ = &job;
-        n_ = nchunks;
-        next_.store(0);
-        busy_ = (int)workers_.size();
-        ++gen_;
-        lk.unlock();
-        cv_.notify_all();
-        for (long c; (c = next_.fetch_add(1)) < nchunks;) job(c);
-        lk.lock();
-        done_.wait(lk, [this] { return busy_ == 0; });
-        job_ = nullptr;
-    }
+// A trajectory slice holds [2][batch][N] doubles (field, instance, point);
+// point q in [0, 2N) of instance b is element (q / N) * BN + b * N + q % N.
+constexpr int LCH = RT;  // points per LDS chunk of k_loop_dot (one per thread)
+constexpr int LMAXE = 3 * RT;  // entries per reduction (three per thread)
+static_assert(AND_MAX + AND_MAX * (AND_MAX + 1) / 2 <= LMAXE, "Gram entries exceed the reduction");
 
-private:
-    HostPool() : pid_(getpid())
-    {
-        const unsigned hc = std::thread::hardware_concurrency();
-        const unsigned t = std::min(16u, hc ? hc : 1u);
-        for (unsigned i = 1; i < t; ++i) workers_.emplace_back([this] { work(); });
-    }
-    ~HostPool()
-    {
-        {
-            std::lock_guard<std::mutex> g(m_);
-            stop_ = true;
-            ++gen_;
-        }
-        cv_.notify_all();
-        for (auto &w : workers_) w.join();
-    }
-    void work()
-    {
-        unsigned long seen = 0;
-        std::unique_lock<std::mutex> lk(m_);
-        for (;;) {
-            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-            if (stop_) return;
-            seen = gen_;
-            const std::function<void(long)> *job = job_;
-            const long n = n_;
-            lk.unlock();
-            for (long c; (c = next_.fetch_add(1)) < n;) (*job)(c);
-            lk.lock();
-            if (--busy_ == 0) done_.notify_one();
-        }
-    }
-    std::vector<std::thread> workers_;
-    std::mutex run_m_;  // held by the thread whose job the pool runs
-    std::mutex m_;
-    std::condition_variable cv_, done_;
-    const std::function<void(long)> *job_ = nullptr;
-    std::atomic<long> next_{0};
-    long n_ = 0;
-    int busy_ = 0;
-    unsigned long gen_ = 0;
-    bool stop_ = false;
-    pid_t pid_;
+enum LoopDotMode { LD_GRAM = 0, LD_CBAR = 1, LD_LOSS = 2, LD_MEAN = 3 };
+
+struct LoopDotArgs {
+    const double *x0;   // GRAM: d_k; CBAR: wbar; LOSS: phi; MEAN: the field
+    const double *dd;   // residual slices (GRAM, CBAR)
+    const double *wt;   // centered-field slices (CBAR)
+    const double *tgt;  // LOSS: target [batch][N]
+    long k;             // current slice (GRAM, CBAR)
+    int mp;             // history length (GRAM, CBAR)
+    double lam;         // mixing fraction (CBAR)
+    long N, BN;
+    int nblk;  // blocks per instance
+    int ne;    // entries
 };
 
-// [0, n) in fixed chunks of HOST_CHUNK points: f(lo, hi) per chunk.  Sums
-// combine per-chunk partials in chunk order, so results do not depend on the
-// number of threads.
-constexpr long HOST_CHUNK = 16384;
-template <typename F> void host_for(long n, F &&f)
+// Per-block partial sums part[(b * nblk + blk) * ne + e] of the entries:
+//   GRAM: e < mp: V_i = <D_i, d_k>; then U_ij = <D_i, D_j>, 1 <= i <= j <= mp
+//         in row order, D_i = d_k - d_{k-i}
+//   CBAR: <wbar, (wt_{k-i} - wt_k) + lam (d_{k-i} - d_k)>, i = 1..mp
+//   LOSS: sum over the A field of (phi - target)^2
+//   MEAN: the sums of field A and of field B
+// over both fields of instance b.  Each block stages LCH points at a time in
+// LDS, every thread accumulates its entries over the chunk, and the partials
+// combine in block order (k_loop_dot_final): the same bits on every run.
+template <int MODE>
+__global__ void __launch_bounds__(RT) k_loop_dot(LoopDotArgs a, double *__restrict__ part)
 {
-    const long nc = (n + HOST_CHUNK - 1) / HOST_CHUNK;
-    HostPool::get().run(nc, [&](long c) { f(c * HOST_CHUNK, std::min(n, (c + 1) * HOST_CHUNK)); });
-}
-// f(lo, hi, acc) adds its chunk's terms to acc[0..width); returns the totals
-template <typename F> std::vector<double> host_sum(long n, int width, F &&f)
-{
-    const long nc = (n + HOST_CHUNK - 1) / HOST_CHUNK;
-    std::vector<double> part((size_t)nc * width, 0.0);
-    HostPool::get().run(nc, [&](long c) {
-        f(c * HOST_CHUNK, std::min(n, (c + 1) * HOST_CHUNK), part.data() + (size_t)c * width);
-    });
-    std::vector<double> tot(width, 0.0);
-    for (long c = 0; c < nc; ++c)
-        for (int j = 0; j < width; ++j) tot[j] += part[(size_t)c * width + j];
-    return tot;
-}
-
-// K host slices of n doubles in one 2 MB-aligned block, advised for
-// transparent huge pages and zeroed by the pool up front, so that the loop
-// adjoint's passes do not take first-touch page faults (4 KB each) on the
-// trajectory they write.
-class HostSlices {
-public:
-    struct Ref {
-        double *p;
-        double *data() const { return p; }
-        double &operator[](size_t i) const { return p[i]; }
-    };
-    HostSlices(size_t k, size_t n) : n_(n)
-    {
-        const size_t align = (size_t)2 << 20;
-        bytes_ = ((k * n * sizeof(double)) + align - 1) / align * align;
-        base_ = bytes_ ? static_cast<double *>(std::aligned_alloc(align, bytes_)) : nullptr;
-        if (!base_) return;
-        (void)madvise(base_, bytes_, MADV_HUGEPAGE);
-        double *b = base_;
-        host_for((long)(bytes_ / sizeof(double)), [b](long lo, long hi) {
-            std::memset(b + lo, 0, sizeof(double) * (hi - lo));
-        });
+    constexpr int XR = (MODE == LD_GRAM || MODE == LD_CBAR) ? AND_MAX + 1 : (MODE == LD_MEAN ? 3 : 1);
+    __shared__ double X[XR][LCH + 1];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const int nx = (MODE == LD_GRAM || MODE == LD_CBAR) ? a.mp + 1 : XR;
+    int ia[3], ib[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int e = tid + s * RT;
+        ia[s] = -1;
+        ib[s] = 0;
+        if (e >= a.ne) continue;
+        if (MODE == LD_GRAM) {
+            if (e < a.mp) {
+                ia[s] = 0;
+                ib[s] = e + 1;
+            } else {
+                int t = e - a.mp, i = 1;
+                while (t >= a.mp - i + 1) {
+                    t -= a.mp - i + 1;
+                    ++i;
+                }
+                ia[s] = i;
+                ib[s] = i + t;
+            }
+        } else if (MODE == LD_CBAR) {
+            ia[s] = 0;
+            ib[s] = e + 1;
+        } else if (MODE == LD_LOSS) {
+            ia[s] = 0;
+            ib[s] = 0;
+        } else {
+            ia[s] = e;
+            ib[s] = 2;
+        }
     }
-    ~HostSlices() { std::free(base_); }
-    HostSlices(const HostSlices &) = delete;
-    HostSlices &operator=(const HostSlices &) = delete;
-    bool ok() const { return base_ != nullptr; }
-    Ref operator[](size_t k) const { return Ref{base_ + k * n_}; }
+    double acc[3] = {0.0, 0.0, 0.0};
+    const long n2 = 2 * a.N;
+    const size_t S = 2 * (size_t)a.BN;
+    for (long c0 = (long)blockIdx.x * LCH; c0 < n2; c0 += (long)a.nblk * LCH) {
+        const long q = c0 + tid;
+        if (q < n2) {
+            const long f = q / a.N, p = q - f * a.N;
+            const size_t idx = (size_t)f * a.BN + (size_t)b * a.N + p;
+            if (MODE == LD_GRAM) {
+                const double d0 = a.x0[idx];
+                X[0][tid] = d0;
+                for (int i = 1; i <= a.mp; ++i) X[i][tid] = d0 - a.dd[(size_t)(a.k - i) * S + idx];
+            } else if (MODE == LD_CBAR) {
+                X[0][tid] = a.x0[idx];
+                const double w0 = a.wt[(size_t)a.k * S + idx], d0 = a.dd[(size_t)a.k * S + idx];
+                for (int i = 1; i <= a.mp; ++i)
+                    X[i][tid] = (a.wt[(size_t)(a.k - i) * S + idx] - w0) + a.lam * (a.dd[(size_t)(a.k - i) * S + idx] - d0);
+            } else if (MODE == LD_LOSS) {
+                X[0][tid] = f == 0 ? a.x0[idx] - a.tgt[(size_t)b * a.N + p] : 0.0;
+            } else {
+                const double v = a.x0[idx];
+                X[0][tid] = f == 0 ? v : 0.0;
+                X[1][tid] = f == 1 ? v : 0.0;
+                X[2][tid] = 1.0;
+            }
+        } else {
+            for (int i = 0; i < nx; ++i) X[i][tid] = 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            if (ia[s] < 0) continue;
+            double t = 0.0;
+            for (int j = 0; j < LCH; ++j) t += X[ia[s]][j] * X[ib[s]][j];
+            acc[s] += t;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+        if (ia[s] >= 0) part[((size_t)b * a.nblk + blockIdx.x) * a.ne + tid + s * RT] = acc[s];
+}
 
-private:
-    double *base_ = nullptr;
-    size_t n_ = 0, bytes_ = 0;
-};
-
-void center(double *w, long N)
+// out[b * ne + e] = sum over blocks of the partials, in block order
+__global__ void __launch_bounds__(RT) k_loop_dot_final(const double *__restrict__ part, int nblk, int ne,
+                                                       double *__restrict__ out)
 {
-    const double s = host_sum(N, 1, [w](long lo, long hi, double *a) {
-        for (long r = lo; r < hi; ++r) a[0] += w[r];
-    })[0];
-    const double m = s / (double)N;
-    host_for(N, [w, m](long lo, long hi) {
-        for (long r = lo; r < hi; ++r) w[r] -= m;
-    });
+    const int b = blockIdx.x;
+    for (int e = threadIdx.x; e < ne; e += RT) {
+        double s = 0.0;
+        for (int k = 0; k < nblk; ++k) s += part[((size_t)b * nblk + k) * ne + e];
+        out[(size_t)b * ne + e] = s;
+    }
+}
+
+// element i of a slice: field f = i / BN, instance (i % BN) / N
+__device__ __forceinline__ long loop_inst(long i, long N, long BN) { return (i - (i / BN) * BN) / N; }
+
+// dst = src minus the mean of its (field, instance); sums [batch][2] from LD_MEAN
+__global__ void __launch_bounds__(RT) k_loop_center(const double *__restrict__ src, double *__restrict__ dst,
+                                                    const double *__restrict__ sums, long N, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    const long f = i / BN;
+    dst[i] = src[i] - sums[2 * loop_inst(i, N, BN) + f] / (double)N;
+}
+
+// chain input u = (wtA + h, wtB)
+__global__ void __launch_bounds__(RT) k_loop_u(const double *__restrict__ wt, const double *__restrict__ h,
+                                               double *__restrict__ u, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    u[i] = i < BN ? wt[i] + h[i] : wt[i];
+}
+
+// residual d = R(wt, phi)
+__global__ void __launch_bounds__(RT) k_loop_resid(const double *__restrict__ wt, const double *__restrict__ ph,
+                                                   double *__restrict__ d, double chiN, double wcmp, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= BN) return;
+    const double a = wt[i], c = wt[BN + i], pa = ph[i], pb = ph[BN + i];
+    const double eta = (a + c - chiN) / 2, psum = 1.0 - pa - pb;
+    d[i] = chiN * pb + eta - a - wcmp * psum;
+    d[BN + i] = chiN * pa + eta - c - wcmp * psum;
+}
+
+// w = wt_k + sum_i c_i (wt_{k-i} - wt_k) + lam (d_k + sum_i c_i (d_{k-i} - d_k));
+// coef [batch][AND_MAX]
+__global__ void __launch_bounds__(RT) k_loop_mix(const double *__restrict__ WT, const double *__restrict__ DD, long k,
+                                                 int mp, const double *__restrict__ coef, double lam,
+                                                 double *__restrict__ w, long N, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    const size_t S = 2 * (size_t)BN;
+    const double *c = coef + loop_inst(i, N, BN) * AND_MAX;
+    const double w0 = WT[(size_t)k * S + i], d0 = DD[(size_t)k * S + i];
+    double wn = w0, dn = d0;
+    for (int j = 1; j <= mp; ++j) {
+        wn += c[j - 1] * (WT[(size_t)(k - j) * S + i] - w0);
+        dn += c[j - 1] * (DD[(size_t)(k - j) * S + i] - d0);
+    }
+    w[i] = wn + lam * dn;
+}
+
+// seed of the reverse sweep: pbar = (2 (phiA - target), 0)
+__global__ void __launch_bounds__(RT) k_loop_pbar0(const double *__restrict__ ph, const double *__restrict__ tgt,
+                                                   double *__restrict__ pbar, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    pbar[i] = i < BN ? 2.0 * (ph[i] - tgt[i]) : 0.0;
+}
+
+// adjoint of k_loop_mix with c held fixed: wbar into Wtbar and Dbar of k and its history
+__global__ void __launch_bounds__(RT) k_loop_mix_adj(const double *__restrict__ wbar, double *__restrict__ WTB,
+                                                     double *__restrict__ DB, long k, int mp,
+                                                     const double *__restrict__ coef, double lam, long N, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    const size_t S = 2 * (size_t)BN;
+    const double *c = coef + loop_inst(i, N, BN) * AND_MAX;
+    double sc = 0.0;
+    for (int j = 0; j < mp; ++j) sc += c[j];
+    const double wb = wbar[i];
+    WTB[(size_t)k * S + i] += (1.0 - sc) * wb;
+    DB[(size_t)k * S + i] += lam * (1.0 - sc) * wb;
+    for (int j = 1; j <= mp; ++j) {
+        WTB[(size_t)(k - j) * S + i] += c[j - 1] * wb;
+        DB[(size_t)(k - j) * S + i] += lam * c[j - 1] * wb;
+    }
+}
+
+// least-squares part of the Anderson adjoint: Ubar_ij = -Vbar_i c_j and the
+// ridge term give Dbar_i = Vbar_i (d_k - S1) - c_i S2 + 2 rdiag Delta_i with
+// S1 = sum_j c_j Delta_j, S2 = sum_j Vbar_j Delta_j; rv [batch][AND_MAX + 1]
+// holds Vbar and, last, rdiag (zeros for an instance whose solve failed)
+__global__ void __launch_bounds__(RT) k_loop_dbar(const double *__restrict__ DD, double *__restrict__ DB, long k,
+                                                  int mp, const double *__restrict__ coef,
+                                                  const double *__restrict__ rv, long N, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    const size_t S = 2 * (size_t)BN;
+    const long b = loop_inst(i, N, BN);
+    const double *c = coef + b * AND_MAX, *vb = rv + b * (AND_MAX + 1);
+    const double rdiag = vb[AND_MAX];
+    const double d0 = DD[(size_t)k * S + i];
+    double S1 = 0.0, S2 = 0.0;
+    for (int j = 1; j <= mp; ++j) {
+        const double Dj = d0 - DD[(size_t)(k - j) * S + i];
+        S1 += c[j - 1] * Dj;
+        S2 += vb[j - 1] * Dj;
+    }
+    double acc0 = 0.0;
+    for (int j = 1; j <= mp; ++j) {
+        const double Dj = d0 - DD[(size_t)(k - j) * S + i];
+        const double Dbj = vb[j - 1] * (d0 - S1) - c[j - 1] * S2 + 2.0 * rdiag * Dj;
+        acc0 += vb[j - 1] * Dj + Dbj;
+        DB[(size_t)(k - j) * S + i] -= Dbj;
+    }
+    DB[(size_t)k * S + i] += acc0;
+}
+
+// residual's linear adjoint: Dbar_k into Wtbar_k and pbar (pbar assigned)
+__global__ void __launch_bounds__(RT) k_loop_resid_adj(const double *__restrict__ DBk, double *__restrict__ WTBk,
+                                                       double *__restrict__ pbar, double chiN, double wcmp, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= BN) return;
+    const double da = DBk[i], db = DBk[BN + i];
+    WTBk[i] += -0.5 * da + 0.5 * db;
+    WTBk[BN + i] += 0.5 * da - 0.5 * db;
+    pbar[i] = wcmp * da + (chiN + wcmp) * db;
+    pbar[BN + i] = (chiN + wcmp) * da + wcmp * db;
+}
+
+// ubar = dL/du of the chain: into hbar (A field) and Wtbar_k
+__global__ void __launch_bounds__(RT) k_loop_ubar(const double *__restrict__ ubar, double *__restrict__ hbar,
+                                                  double *__restrict__ WTBk, long BN)
+{
+    const long i = (long)blockIdx.x * RT + threadIdx.x;
+    if (i >= 2 * BN) return;
+    WTBk[i] += ubar[i];
+    if (i < BN) hbar[i] += ubar[i];
+}
+
+__global__ void __launch_bounds__(RT) k_loop_zero(double *__restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * RT + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// device memory of one scft_adjoint_loop call, kept in the context (grown on demand)
+int ensure_loop_buffer(scft_ctx *c, size_t bytes)
+{
+    if (c->loop_bytes >= bytes) return SCFT_OK;
+    if (c->loop) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(c->loop);
+        c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->loop), c->allocs.end());
+        c->device_bytes -= (long)c->loop_bytes;
+        c->loop = nullptr;
+        c->loop_bytes = 0;
+    }
+    int rc = dalloc(c, (void **)&c->loop, bytes);
+    if (rc) return rc;
+    c->loop_bytes = bytes;
+    return SCFT_OK;
 }
 
 }  // namespace
@@ -2523,115 +2660,117 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     if (rc) return rc;
     const long N = ctx->N;
     const int nb = ctx->batch;
-    const size_t BN = (size_t)nb * N;
+    const long BN = (long)nb * N;
+    const size_t S = 2 * (size_t)BN;  // doubles per trajectory slice
     const double chiN = ctx->p.chiN, wcmp = ctx->p.wcmp;
     const bool anderson = ctx->p.mixing == SCFT_MIX_ANDERSON;
     const double ridge = ctx->p.anderson_reg;
     const int m = anderson ? ctx->p.anderson_m : 0;
     const double lam = anderson ? ctx->p.anderson_mix : ctx->p.wopt;
     const int K = n_iter;
-    // per-iteration state (FP64): centered fields, phi, residual
-    HostSlices wt(K, 2 * BN), phi(K, 2 * BN), dd(K, 2 * BN);
-    if (!wt.ok() || !phi.ok() || !dd.ok())
-        return set_err(ctx, SCFT_E_NOMEM, "scft_adjoint_loop: host trajectory allocation failed");
-    // Anderson per update k, per instance: mp, coefficients, U, solve status
-    std::vector<int> mpk(K, 0);
-    std::vector<std::vector<double>> ck(K), Uk(K);
-    std::vector<std::vector<char>> okk(K);
-    double *dev = ctx->adj;
-    auto inst = [&](auto &&v, int f, int b) { return v.data() + (size_t)f * BN + (size_t)b * N; };
-    // pinned staging for the per-iteration transfers ([2][BN] each), held for this call
+    const bool grad = grad_h != nullptr;
+    const int nbl = (int)std::min<long>(256, (2 * N + LCH - 1) / LCH);  // reduction blocks per instance
+    hipStream_t s = ctx->stream;
+    // device layout (doubles): WT, PH, DD [K][S]; WTB, DB [K][S] (gradient only);
+    // w, wbar [S]; hbar, h, target [BN]; partials; sums; coefficients; Vbar/rdiag; Q
+    const size_t n_traj = (size_t)K * S;
+    const size_t n_part = (size_t)nbl * nb * LMAXE, n_red = (size_t)nb * LMAXE;
+    const size_t n_coef = (size_t)K * nb * AND_MAX, n_rv = (size_t)K * nb * (AND_MAX + 1), n_q = (size_t)K * nb;
+    const size_t total = (grad ? 5 : 3) * n_traj + 2 * S + 3 * (size_t)BN + n_part + n_red + n_coef + n_rv + n_q;
+    if ((rc = ensure_loop_buffer(ctx, total * sizeof(double)))) return rc;
+    double *p = ctx->loop;
+    auto take = [&p](size_t n) {
+        double *r = p;
+        p += n;
+        return r;
+    };
+    double *WT = take(n_traj), *PH = take(n_traj), *DD = take(n_traj);
+    double *WTB = grad ? take(n_traj) : nullptr, *DB = grad ? take(n_traj) : nullptr;
+    double *wd = take(S), *wbar = take(S), *hbar = take(BN), *hd = take(BN), *tg = take(BN);
+    double *part = take(n_part), *red = take(n_red), *coefd = take(n_coef), *rvd = take(n_rv), *Qk = take(n_q);
+    // pinned host side of the small transfers: sums, coefficients, Vbar/rdiag, Q
     struct Pinned {
         double *p = nullptr;
         ~Pinned()
         {
             if (p) (void)hipHostFree(p);
         }
-    } st0, st1;
-    HIPX(ctx, hipHostMalloc((void **)&st0.p, 16 * BN, hipHostMallocDefault));
-    HIPX(ctx, hipHostMalloc((void **)&st1.p, 16 * BN, hipHostMallocDefault));
-    auto copy_par = [](double *dst, const double *src, size_t n) {
-        host_for((long)n, [&](long lo, long hi) { std::memcpy(dst + lo, src + lo, sizeof(double) * (hi - lo)); });
+    } hp;
+    HIPX(ctx, hipHostMalloc((void **)&hp.p, sizeof(double) * (n_red + n_coef + n_rv + n_q), hipHostMallocDefault));
+    double *h_red = hp.p, *h_coef = h_red + n_red, *h_rv = h_coef + n_coef, *h_q = h_rv + n_rv;
+    std::memset(h_coef, 0, sizeof(double) * (n_coef + n_rv));
+    double *dev = ctx->adj;  // chain scratch: u [0, 2BN), lam = pbar [2BN, 4BN), ubar [4BN, 6BN), phi [6BN, 8BN)
+    const unsigned g1 = (unsigned)((BN + RT - 1) / RT), g2 = (unsigned)((2 * BN + RT - 1) / RT);
+    auto zero = [&](double *x, size_t n) {
+        hipLaunchKernelGGL(k_loop_zero, dim3((unsigned)((n + RT - 1) / RT)), dim3(RT), 0, s, x, n);
     };
-    // u = (w_A + h, w_B) into st0
-    auto stage_u = [&](const double *wp) {
-        double *u = st0.p;
-        host_for((long)BN, [&](long lo, long hi) {
-            for (long i = lo; i < hi; ++i) {
-                u[i] = wp[i] + h[i];
-                u[BN + i] = wp[BN + i];
-            }
-        });
+    LoopDotArgs la{};
+    la.N = N;
+    la.BN = BN;
+    la.nblk = nbl;
+    // sums into red[b * ne + e]
+    auto dot = [&](int mode, int ne) {
+        la.ne = ne;
+        const dim3 g((unsigned)nbl, (unsigned)nb);
+        switch (mode) {
+        case LD_GRAM: hipLaunchKernelGGL(k_loop_dot<LD_GRAM>, g, dim3(RT), 0, s, la, part); break;
+        case LD_CBAR: hipLaunchKernelGGL(k_loop_dot<LD_CBAR>, g, dim3(RT), 0, s, la, part); break;
+        case LD_LOSS: hipLaunchKernelGGL(k_loop_dot<LD_LOSS>, g, dim3(RT), 0, s, la, part); break;
+        default: hipLaunchKernelGGL(k_loop_dot<LD_MEAN>, g, dim3(RT), 0, s, la, part); break;
+        }
+        hipLaunchKernelGGL(k_loop_dot_final, dim3((unsigned)nb), dim3(RT), 0, s, (const double *)part, nbl, ne, red);
     };
-    auto run_conc = [&](const double *w, double *ph) -> int {
-        stage_u(w);
-        HIPX(ctx, hipMemcpyAsync(dev, st0.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
-        HIPX(ctx, enqueue_conc_scratch(ctx));
-        HIPX(ctx, hipMemcpyAsync(st1.p, dev + 6 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
-        std::vector<double> q(nb);
-        HIPX(ctx, hipMemcpyAsync(q.data(), ctx->Q, 8 * nb, hipMemcpyDeviceToHost, ctx->stream));
-        HIPX(ctx, hipStreamSynchronize(ctx->stream));
-        copy_par(ph, st1.p, 2 * BN);
-        for (int b = 0; b < nb; ++b)
-            if (!std::isfinite(q[b]) || !(q[b] > 0.0))
-                return set_err(ctx, SCFT_E_NONFINITE, "Q not finite and > 0 in the adjoint's forward loop");
+    // dst = P src (mean removal per field and instance)
+    auto center_dev = [&](const double *src, double *dst) {
+        la.x0 = src;
+        dot(LD_MEAN, 2);
+        hipLaunchKernelGGL(k_loop_center, dim3(g2), dim3(RT), 0, s, src, dst, (const double *)red, N, BN);
+    };
+    // sums of the last dot() on the host
+    auto fetch = [&](int ne) -> int {
+        HIPX(ctx, hipMemcpyAsync(h_red, red, sizeof(double) * nb * ne, hipMemcpyDeviceToHost, s));
+        HIPX(ctx, hipStreamSynchronize(s));
         return SCFT_OK;
     };
+    HIPX(ctx, hipMemcpyAsync(wd, w0A, 8 * BN, hipMemcpyHostToDevice, s));
+    HIPX(ctx, hipMemcpyAsync(wd + BN, w0B, 8 * BN, hipMemcpyHostToDevice, s));
+    HIPX(ctx, hipMemcpyAsync(hd, h, 8 * BN, hipMemcpyHostToDevice, s));
+    HIPX(ctx, hipMemcpyAsync(tg, target, 8 * BN, hipMemcpyHostToDevice, s));
+    // Anderson per update k, per instance: mp, U, solve status (coefficients in h_coef)
+    std::vector<int> mpk(K, 0);
+    std::vector<std::vector<double>> Uk(K);
+    std::vector<std::vector<char>> okk(K);
     // ---- forward ----
-    std::vector<double> w(2 * BN);
-    std::memcpy(w.data(), w0A, 8 * BN);
-    std::memcpy(w.data() + BN, w0B, 8 * BN);
     for (int k = 0; k < K; ++k) {
-        copy_par(wt[k].data(), w.data(), 2 * BN);
-        for (int b = 0; b < nb; ++b)
-            for (int f = 0; f < 2; ++f) center(inst(wt[k], f, b), N);
-        if ((rc = run_conc(wt[k].data(), phi[k].data()))) return rc;
+        double *wtk = WT + (size_t)k * S, *phk = PH + (size_t)k * S;
+        center_dev(wd, wtk);
+        hipLaunchKernelGGL(k_loop_u, dim3(g2), dim3(RT), 0, s, (const double *)wtk, (const double *)hd, dev, BN);
+        HIPX(ctx, enqueue_conc_scratch(ctx));
+        HIPX(ctx, hipMemcpyAsync(phk, dev + 6 * BN, 8 * S, hipMemcpyDeviceToDevice, s));
+        HIPX(ctx, hipMemcpyAsync(Qk + (size_t)k * nb, ctx->Q, 8 * nb, hipMemcpyDeviceToDevice, s));
         if (k == K - 1) break;
-        {
-            const double *wk = wt[k].data(), *pk = phi[k].data();
-            double *dk = dd[k].data();
-            host_for((long)BN, [&](long lo, long hi) {
-                for (long i = lo; i < hi; ++i) {
-                    const double a = wk[i], c = wk[BN + i], pa = pk[i], pb = pk[BN + i];
-                    const double eta = (a + c - chiN) / 2, psum = 1.0 - pa - pb;
-                    dk[i] = chiN * pb + eta - a - wcmp * psum;
-                    dk[BN + i] = chiN * pa + eta - c - wcmp * psum;
-                }
-            });
-        }
+        double *ddk = DD + (size_t)k * S;
+        hipLaunchKernelGGL(k_loop_resid, dim3(g1), dim3(RT), 0, s, (const double *)wtk, (const double *)phk, ddk, chiN,
+                           wcmp, BN);
         const int mp = anderson ? std::min(k, m) : 0;
         mpk[k] = mp;
-        ck[k].assign((size_t)nb * std::max(mp, 1), 0.0);
-        Uk[k].assign((size_t)nb * std::max(mp * mp, 1), 0.0);
         okk[k].assign(nb, 0);
-        for (int b = 0; b < nb; ++b) {
-            std::vector<double> c(mp, 0.0);
-            if (mp > 0) {
-                // U_ij = <d_k - d_{k-i}, d_k - d_{k-j}>, V_i = <d_k - d_{k-i}, d_k>
-                // one pass over the points accumulates every <Delta_i, Delta_j> and <Delta_i, d_k>
-                std::vector<double> U(mp * mp, 0.0), V(mp, 0.0);
-                for (int f = 0; f < 2; ++f) {
-                    const double *d0 = inst(dd[k], f, b);
-                    std::vector<const double *> dk(mp + 1);
-                    for (int i = 1; i <= mp; ++i) dk[i] = inst(dd[k - i], f, b);
-                    // per chunk: acc[0..mp) = V, acc[mp + i*mp + j] = U_ij (j >= i)
-                    const std::vector<double> t = host_sum(N, mp + mp * mp, [&](long lo, long hi, double *acc) {
-                        double Dl[AND_MAX];
-                        for (long r = lo; r < hi; ++r) {
-                            for (int i = 0; i < mp; ++i) Dl[i] = d0[r] - dk[i + 1][r];
-                            for (int i = 0; i < mp; ++i) {
-                                acc[i] += Dl[i] * d0[r];
-                                for (int j = i; j < mp; ++j) acc[mp + i * mp + j] += Dl[i] * Dl[j];
-                            }
-                        }
-                    });
-                    for (int i = 0; i < mp; ++i) {
-                        V[i] += t[i];
-                        for (int j = i; j < mp; ++j) U[i * mp + j] += t[mp + i * mp + j];
-                    }
-                }
-                for (int i = 0; i < mp; ++i)
-                    for (int j = 0; j < i; ++j) U[i * mp + j] = U[j * mp + i];
+        double *hc = h_coef + (size_t)k * nb * AND_MAX;
+        if (mp > 0) {
+            // U_ij = <d_k - d_{k-i}, d_k - d_{k-j}>, V_i = <d_k - d_{k-i}, d_k>
+            const int ne = mp + mp * (mp + 1) / 2;
+            la.x0 = ddk;
+            la.dd = DD;
+            la.k = k;
+            la.mp = mp;
+            dot(LD_GRAM, ne);
+            if ((rc = fetch(ne))) return rc;
+            Uk[k].assign((size_t)nb * mp * mp, 0.0);
+            for (int b = 0; b < nb; ++b) {
+                const double *t = h_red + (size_t)b * ne;
+                std::vector<double> U(mp * mp), V(t, t + mp);
+                for (int i = 0, e = mp; i < mp; ++i)
+                    for (int j = i; j < mp; ++j, ++e) U[i * mp + j] = U[j * mp + i] = t[e];
                 if (ridge > 0.0) {  // U += ridge * tr(U)/mp * I, as k_and_solve
                     double tr = 0.0;
                     for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
@@ -2640,182 +2779,98 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
                 std::vector<double> x = V;
                 const bool ok = host_solve(mp, U, x);
                 okk[k][b] = ok;
-                if (ok) c = x;
+                for (int i = 0; i < mp; ++i) hc[(size_t)b * AND_MAX + i] = ok ? x[i] : 0.0;
                 std::copy(U.begin(), U.end(), Uk[k].begin() + (size_t)b * mp * mp);
-                std::copy(c.begin(), c.end(), ck[k].begin() + (size_t)b * mp);
             }
-            for (int f = 0; f < 2; ++f) {
-                const double *w0 = inst(wt[k], f, b), *d0 = inst(dd[k], f, b);
-                double *wo = w.data() + (size_t)f * BN + (size_t)b * N;
-                std::vector<const double *> wi(mp + 1), di(mp + 1);
-                for (int i = 1; i <= mp; ++i) {
-                    wi[i] = inst(wt[k - i], f, b);
-                    di[i] = inst(dd[k - i], f, b);
-                }
-                host_for(N, [&](long lo, long hi) {
-                    for (long r = lo; r < hi; ++r) {
-                        double wn = w0[r], dn = d0[r];
-                        for (int i = 1; i <= mp; ++i) {
-                            wn += c[i - 1] * (wi[i][r] - w0[r]);
-                            dn += c[i - 1] * (di[i][r] - d0[r]);
-                        }
-                        wo[r] = wn + lam * dn;
-                    }
-                });
-            }
+            HIPX(ctx, hipMemcpyAsync(coefd + (size_t)k * nb * AND_MAX, hc, sizeof(double) * nb * AND_MAX,
+                                     hipMemcpyHostToDevice, s));
         }
+        hipLaunchKernelGGL(k_loop_mix, dim3(g2), dim3(RT), 0, s, (const double *)WT, (const double *)DD, (long)k, mp,
+                           (const double *)(coefd + (size_t)k * nb * AND_MAX), lam, wd, N, BN);
     }
-    for (int b = 0; b < nb; ++b) {
-        const double *pa = inst(phi[K - 1], 0, b), *tg = target + (size_t)b * N;
-        const double L = host_sum(N, 1, [&](long lo, long hi, double *acc) {
-            for (long r = lo; r < hi; ++r) {
-                const double e = pa[r] - tg[r];
-                acc[0] += e * e;
-            }
-        })[0];
-        if (loss) loss[b] = L;
+    HIPX(ctx, hipGetLastError());
+    la.x0 = PH + (size_t)(K - 1) * S;
+    la.tgt = tg;
+    dot(LD_LOSS, 1);
+    if ((rc = fetch(1))) return rc;
+    HIPX(ctx, hipMemcpyAsync(h_q, Qk, 8 * n_q, hipMemcpyDeviceToHost, s));
+    HIPX(ctx, hipStreamSynchronize(s));
+    for (size_t i = 0; i < n_q; ++i)
+        if (!std::isfinite(h_q[i]) || !(h_q[i] > 0.0)) {
+            ctx->have_conc = 0;
+            return set_err(ctx, SCFT_E_NONFINITE, "Q not finite and > 0 in the adjoint's forward loop (iteration " +
+                                                      std::to_string(i / nb) + ")");
+        }
+    if (loss)
+        for (int b = 0; b < nb; ++b) loss[b] = h_red[b];
+    if (phiA_last) {
+        HIPX(ctx, hipMemcpyAsync(phiA_last, PH + (size_t)(K - 1) * S, 8 * BN, hipMemcpyDeviceToHost, s));
+        HIPX(ctx, hipStreamSynchronize(s));
     }
-    if (phiA_last) std::memcpy(phiA_last, phi[K - 1].data(), 8 * BN);
-    if (!grad_h) return SCFT_OK;
+    if (!grad) {
+        ctx->have_conc = 0;
+        return SCFT_OK;
+    }
     // ---- reverse ----
-    HostSlices Wtb(K, 2 * BN), Db(K, 2 * BN);  // zeroed
-    if (!Wtb.ok() || !Db.ok())
-        return set_err(ctx, SCFT_E_NOMEM, "scft_adjoint_loop: host adjoint allocation failed");
-    std::vector<double> wbar(2 * BN, 0.0), hbar(BN, 0.0), pbar(2 * BN), ubar(2 * BN);
+    zero(WTB, n_traj);
+    zero(DB, n_traj);
+    zero(hbar, BN);
+    zero(wbar, S);
+    double *pbar = dev + 2 * BN, *ubar = dev + 4 * BN;
     for (int k = K - 1; k >= 0; --k) {
-        std::fill(pbar.begin(), pbar.end(), 0.0);
+        double *wtbk = WTB + (size_t)k * S, *dbk = DB + (size_t)k * S;
         if (k == K - 1) {
-            const double *pk = phi[k].data();
-            host_for((long)BN, [&](long lo, long hi) {
-                for (long i = lo; i < hi; ++i) pbar[i] = 2.0 * (pk[i] - target[i]);
-            });
+            hipLaunchKernelGGL(k_loop_pbar0, dim3(g2), dim3(RT), 0, s, (const double *)(PH + (size_t)k * S),
+                               (const double *)tg, pbar, BN);
         } else {
             const int mp = mpk[k];
-            for (int b = 0; b < nb; ++b) {
-                const double *c = ck[k].data() + (size_t)b * std::max(mp, 1);
-                double sc = 0.0;
-                for (int i = 0; i < mp; ++i) sc += c[i];
-                for (int f = 0; f < 2; ++f) {
-                    const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
-                    double *wt0 = inst(Wtb[k], f, b), *d0 = inst(Db[k], f, b);
-                    std::vector<double *> wti(mp + 1), di(mp + 1);
-                    for (int i = 1; i <= mp; ++i) {
-                        wti[i] = inst(Wtb[k - i], f, b);
-                        di[i] = inst(Db[k - i], f, b);
-                    }
-                    host_for(N, [&](long lo, long hi) {
-                        for (long r = lo; r < hi; ++r) {
-                            wt0[r] += (1.0 - sc) * wb[r];
-                            d0[r] += lam * (1.0 - sc) * wb[r];
-                            for (int i = 1; i <= mp; ++i) {
-                                wti[i][r] += c[i - 1] * wb[r];
-                                di[i][r] += lam * c[i - 1] * wb[r];
-                            }
-                        }
-                    });
-                }
-                if (mp > 0 && okk[k][b]) {
-                    // cbar_i = <wbar, (wt_{k-i} - wt_k) + lam (d_{k-i} - d_k)>
-                    std::vector<double> cbar(mp, 0.0);
-                    for (int f = 0; f < 2; ++f) {
-                        const double *wb = wbar.data() + (size_t)f * BN + (size_t)b * N;
-                        const double *w0 = inst(wt[k], f, b), *d0 = inst(dd[k], f, b);
-                        std::vector<const double *> wi(mp + 1), di(mp + 1);
-                        for (int i = 1; i <= mp; ++i) {
-                            wi[i] = inst(wt[k - i], f, b);
-                            di[i] = inst(dd[k - i], f, b);
-                        }
-                        const std::vector<double> t = host_sum(N, mp, [&](long lo, long hi, double *acc) {
-                            for (long r = lo; r < hi; ++r)
-                                for (int i = 1; i <= mp; ++i)
-                                    acc[i - 1] += wb[r] * ((wi[i][r] - w0[r]) + lam * (di[i][r] - d0[r]));
-                        });
-                        for (int i = 0; i < mp; ++i) cbar[i] += t[i];
-                    }
-                    std::vector<double> U(Uk[k].begin() + (size_t)b * mp * mp,
-                                          Uk[k].begin() + (size_t)(b + 1) * mp * mp);
-                    std::vector<double> Vb = cbar;  // U symmetric: Vbar = U^-1 cbar
-                    host_solve(mp, U, Vb);
+            const double *cd = coefd + (size_t)k * nb * AND_MAX;
+            hipLaunchKernelGGL(k_loop_mix_adj, dim3(g2), dim3(RT), 0, s, (const double *)wbar, WTB, DB, (long)k, mp, cd,
+                               lam, N, BN);
+            if (mp > 0) {
+                // cbar_i = <wbar, (wt_{k-i} - wt_k) + lam (d_{k-i} - d_k)>
+                la.x0 = wbar;
+                la.wt = WT;
+                la.dd = DD;
+                la.k = k;
+                la.mp = mp;
+                la.lam = lam;
+                dot(LD_CBAR, mp);
+                if ((rc = fetch(mp))) return rc;
+                const double *hc = h_coef + (size_t)k * nb * AND_MAX;
+                double *hr = h_rv + (size_t)k * nb * (AND_MAX + 1);
+                for (int b = 0; b < nb; ++b) {
+                    if (!okk[k][b]) continue;  // c = 0: no least-squares term
+                    std::vector<double> U(Uk[k].begin() + (size_t)b * mp * mp, Uk[k].begin() + (size_t)(b + 1) * mp * mp);
+                    std::vector<double> Vb(h_red + (size_t)b * mp, h_red + (size_t)(b + 1) * mp);
+                    host_solve(mp, U, Vb);  // U symmetric: Vbar = U^-1 cbar
                     // the ridge ridge*tr(U)/mp*I adds -(ridge/mp)(Vbar.c) I to Ubar
                     double rdiag = 0.0;
                     if (ridge > 0.0)
-                        for (int i = 0; i < mp; ++i) rdiag -= ridge / mp * Vb[i] * c[i];
-                    // Ubar_ij = -Vbar_i c_j;  Dbar_i = sum_j (Ubar_ij + Ubar_ji) Delta_j + Vbar_i d_k
-                    // Dbar_i = Vbar_i (d_k - S1) - c_i S2 + 2 rdiag Delta_i with
-                    // S1 = sum_j c_j Delta_j, S2 = sum_j Vbar_j Delta_j (O(m) per point)
-                    for (int f = 0; f < 2; ++f) {
-                        const double *d0 = inst(dd[k], f, b);
-                        double *db0 = inst(Db[k], f, b);
-                        std::vector<const double *> dk(mp + 1);
-                        std::vector<double *> dbk(mp + 1);
-                        for (int i = 1; i <= mp; ++i) {
-                            dk[i] = inst(dd[k - i], f, b);
-                            dbk[i] = inst(Db[k - i], f, b);
-                        }
-                        host_for(N, [&](long lo, long hi) {
-                            for (long r = lo; r < hi; ++r) {
-                                double S1 = 0.0, S2 = 0.0;
-                                for (int j = 1; j <= mp; ++j) {
-                                    const double Dj = d0[r] - dk[j][r];
-                                    S1 += c[j - 1] * Dj;
-                                    S2 += Vb[j - 1] * Dj;
-                                }
-                                double acc0 = 0.0;
-                                for (int i = 1; i <= mp; ++i) {
-                                    const double Di = d0[r] - dk[i][r];
-                                    const double Dbi = Vb[i - 1] * (d0[r] - S1) - c[i - 1] * S2 + 2.0 * rdiag * Di;
-                                    acc0 += Vb[i - 1] * Di + Dbi;
-                                    dbk[i][r] -= Dbi;
-                                }
-                                db0[r] += acc0;
-                            }
-                        });
-                    }
+                        for (int i = 0; i < mp; ++i) rdiag -= ridge / mp * Vb[i] * hc[(size_t)b * AND_MAX + i];
+                    for (int i = 0; i < mp; ++i) hr[(size_t)b * (AND_MAX + 1) + i] = Vb[i];
+                    hr[(size_t)b * (AND_MAX + 1) + AND_MAX] = rdiag;
                 }
+                double *rd = rvd + (size_t)k * nb * (AND_MAX + 1);
+                HIPX(ctx, hipMemcpyAsync(rd, hr, sizeof(double) * nb * (AND_MAX + 1), hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_loop_dbar, dim3(g2), dim3(RT), 0, s, (const double *)DD, DB, (long)k, mp, cd,
+                                   (const double *)rd, N, BN);
             }
             // residual d_k = R(wt_k, phi_k): linear adjoint
-            {
-                const double *dbk = Db[k].data();
-                double *wtb = Wtb[k].data();
-                host_for((long)BN, [&](long lo, long hi) {
-                    for (long i = lo; i < hi; ++i) {
-                        const double da = dbk[i], db = dbk[BN + i];
-                        wtb[i] += -0.5 * da + 0.5 * db;
-                        wtb[BN + i] += 0.5 * da - 0.5 * db;
-                        pbar[i] += wcmp * da + (chiN + wcmp) * db;
-                        pbar[BN + i] += (chiN + wcmp) * da + wcmp * db;
-                    }
-                });
-            }
+            hipLaunchKernelGGL(k_loop_resid_adj, dim3(g1), dim3(RT), 0, s, (const double *)dbk, wtbk, pbar, chiN, wcmp,
+                               BN);
         }
-        // chain adjoint on the device: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
-        {
-            stage_u(wt[k].data());
-            copy_par(st1.p, pbar.data(), 2 * BN);
-            HIPX(ctx, hipMemcpyAsync(dev, st0.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
-            HIPX(ctx, hipMemcpyAsync(dev + 2 * BN, st1.p, 16 * BN, hipMemcpyHostToDevice, ctx->stream));
-            HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0) : enqueue_conc_adjoint<double>(ctx, 0.0));
-            // into st0: its upload precedes this copy on the stream
-            HIPX(ctx, hipMemcpyAsync(st0.p, dev + 4 * BN, 16 * BN, hipMemcpyDeviceToHost, ctx->stream));
-            HIPX(ctx, hipStreamSynchronize(ctx->stream));
-            copy_par(ubar.data(), st0.p, 2 * BN);
-        }
-        {
-            double *wtb = Wtb[k].data();
-            host_for((long)BN, [&](long lo, long hi) {
-                for (long i = lo; i < hi; ++i) {
-                    hbar[i] += ubar[i];
-                    wtb[i] += ubar[i];
-                    wtb[BN + i] += ubar[BN + i];
-                }
-            });
-        }
+        // chain adjoint: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
+        hipLaunchKernelGGL(k_loop_u, dim3(g2), dim3(RT), 0, s, (const double *)(WT + (size_t)k * S), (const double *)hd,
+                           dev, BN);
+        HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0) : enqueue_conc_adjoint<double>(ctx, 0.0));
+        hipLaunchKernelGGL(k_loop_ubar, dim3(g2), dim3(RT), 0, s, (const double *)ubar, hbar, wtbk, BN);
         // mean removal P is symmetric: wbar(k-1) = P Wtbar(k)
-        copy_par(wbar.data(), Wtb[k].data(), 2 * BN);
-        for (int b = 0; b < nb; ++b)
-            for (int f = 0; f < 2; ++f) center(wbar.data() + (size_t)f * BN + (size_t)b * N, N);
+        if (k > 0) center_dev(wtbk, wbar);
     }
-    std::memcpy(grad_h, hbar.data(), 8 * BN);
+    HIPX(ctx, hipGetLastError());
+    HIPX(ctx, hipMemcpyAsync(grad_h, hbar, 8 * BN, hipMemcpyDeviceToHost, s));
+    HIPX(ctx, hipStreamSynchronize(s));
     ctx->have_conc = 0;
     return SCFT_OK;
 }

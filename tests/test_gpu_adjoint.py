@@ -142,8 +142,8 @@ def test_adjoint_loop_fp32_vs_fp64(pkg):
 
 @pytest.mark.parametrize("mixing,ridge", [(0, 0.0), (1, 0.0), (1, 1e-3)])
 def test_adjoint_loop_forward_matches_iterate(pkg, mixing, ridge):
-    """The adjoint's forward (host mixing, its own least squares and ridge)
-    follows the solver's own loop on the device."""
+    """The adjoint's forward (its own mixing kernels, least squares and
+    ridge) follows the solver's own loop."""
     n = 12
     p = _params(pkg, (n, n, n), 4.0, 18.0, 0.5, 0.02, mixing=mixing, anderson_m=4, anderson_mix=0.2,
                 anderson_reg=ridge)
@@ -159,6 +159,73 @@ def test_adjoint_loop_forward_matches_iterate(pkg, mixing, ridge):
         wB = wB[0] - wB[0].mean()
         _, _, _, pa, _, _ = s.conc_adjoint(wA, wB)
     assert _rel(phi_last[0], pa[0]) < (1e-11 if mixing == 0 else 1e-9)
+
+
+@pytest.mark.parametrize("mixing", [0, 1])
+def test_adjoint_loop_batch_matches_single(pkg, mixing):
+    """Two instances in one context (their own start fields, h and target)
+    give each instance's loss and gradient as a context of one would: the
+    per-instance sums, least squares and chain adjoint do not mix instances."""
+    p1, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing)
+    n = p1.nx
+    v0A, v0B = pkg.fields.seeded_fields(n, n, n, 4.0, 4.0, 4.0, "lamellar", 0.7, 0.15, 91)
+    h2 = 0.05 * pkg.fields.splitmix_uniform(92, N)
+    t2 = 0.5 + 0.2 * np.sin(2 * np.pi * pkg.fields.coords(n, n, n, 4.0, 4.0, 4.0)[1] / 4.0)
+    single = []
+    for a, b, hh, tt in ((w0A, w0B, h, target), (v0A, v0B, h2, t2)):
+        with pkg.Solver(p1) as s:
+            single.append(s.adjoint_loop(hh, a, b, iters, tt))
+    p2, _, _, _, _, _, _ = _loop_case(pkg, mixing, batch=2)
+    with pkg.Solver(p2) as s:
+        loss, g, _ = s.adjoint_loop(np.concatenate([h, h2]), np.concatenate([w0A, v0A]), np.concatenate([w0B, v0B]),
+                                    iters, np.concatenate([target, t2]))
+    for b in range(2):
+        dl = abs(loss[b] - single[b][0][0]) / abs(single[b][0][0])
+        dg = _rel(g[b], single[b][1][0])
+        print(f"mixing={mixing} instance {b}: rel dloss={dl:.2e} rel dgrad={dg:.2e}")
+        assert dl < 1e-12 and dg < 1e-10
+
+
+def test_adjoint_loop_deterministic(pkg):
+    """Two calls give the same bits: every device sum runs in a fixed order."""
+    p, N, h, w0A, w0B, target, iters = _loop_case(pkg, 1, anderson_reg=1e-3)
+    with pkg.Solver(p) as s:
+        l1, g1, _ = s.adjoint_loop(h, w0A, w0B, iters, target)
+        l2, g2, _ = s.adjoint_loop(h, w0A, w0B, iters, target)
+    assert np.array_equal(l1, l2) and np.array_equal(g1, g2)
+
+
+def test_adjoint_loop_long_history(pkg):
+    """A 24-deep Anderson history (300 Gram entries per update, more than one
+    per reduction thread): the adjoint's forward follows the solver's own
+    Anderson loop, and its gradient matches central differences."""
+    n, m, iters = 8, 24, 28
+    p = _params(pkg, (n, n, n), 4.0, 16.0, 0.5, 0.02, mixing=1, anderson_m=m, anderson_mix=0.2, anderson_reg=1e-2)
+    N = n ** 3
+    w0A, w0B = pkg.fields.seeded_fields(n, n, n, 4.0, 4.0, 4.0, "lamellar", 0.5, 0.1, 93)
+    x = pkg.fields.coords(n, n, n, 4.0, 4.0, 4.0)[0]
+    target = 0.5 + 0.3 * np.cos(2 * np.pi * x / 4.0)
+    h = 0.02 * pkg.fields.splitmix_uniform(94, N)
+    rng = np.random.default_rng(9)
+    eps = 1e-5
+    with pkg.Solver(p) as s:
+        _, _, phi_last = s.adjoint_loop(np.zeros(N), w0A, w0B, iters, target, grad=False)
+        s.set_fields(w0A, w0B)
+        s.iterate(iters - 1)
+        _, _, wA, wB = s.get_fields()
+        _, _, _, pa, _, _ = s.conc_adjoint(wA[0] - wA[0].mean(), wB[0] - wB[0].mean())
+        dfwd = _rel(phi_last[0], pa[0])
+        loss, g, _ = s.adjoint_loop(h, w0A, w0B, iters, target)
+        worst = 0.0
+        for _ in range(3):
+            v = rng.standard_normal(N)
+            lp = s.adjoint_loop(h + eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+            lm = s.adjoint_loop(h - eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+            fd = (lp - lm) / (2 * eps)
+            ad = float(g[0] @ v)
+            worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-12))
+    print(f"m={m}, {iters} iterations: forward vs iterate {dfwd:.2e}, worst rel FD mismatch {worst:.2e}")
+    assert dfwd < 1e-8 and worst < 1e-4
 
 
 def test_torch_autograd_wrapper(pkg):
