@@ -2236,14 +2236,28 @@ static int time_contour(scft_ctx *ctx, int reps, float *ms)
     return SCFT_OK;
 }
 
+// W factors of the scratch fields u (no mean subtraction) and slice 0
 template <typename T>
-static hipError_t enqueue_conc_adjoint(scft_ctx *c, double mu)
+static hipError_t enqueue_prep_scratch(scft_ctx *c)
+{
+    const dim3 g(c->nblk, c->batch);
+    double *uA = c->adj, *uB = uA + (size_t)c->batch * c->N;
+    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, uA, (const double *)uB, c->N, c->nblk,
+                       (const double *)nullptr, c->p.ds / 2, (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0),
+                       c->batch);
+    return hipGetLastError();
+}
+
+// forward = false: the caller has restored what the forward evaluation of u
+// leaves behind (W factors, the slice store or the checkpoints, Q, phi at pA/pB)
+template <typename T>
+static hipError_t enqueue_conc_adjoint(scft_ctx *c, double mu, bool forward = true)
 {
     using C = typename cplx<T>::type;
     const size_t BN = (size_t)c->batch * c->N;
     double *uA = c->adj, *uB = uA + BN, *lA = uB + BN, *lB = lA + BN, *gA = lB + BN, *gB = gA + BN, *pA = gB + BN,
            *pB = pA + BN, *sc = pB + BN;
-    hipError_t e = enqueue_conc_fields<T>(c, uA, uB, nullptr, pA, pB);
+    hipError_t e = forward ? enqueue_conc_fields<T>(c, uA, uB, nullptr, pA, pB) : hipSuccess;
     if (e != hipSuccess) return e;
     const dim3 g(c->nblk, c->batch);
     hipLaunchKernelGGL(k_dot_lamphi, g, dim3(RT), 0, c->stream, (const double *)lA, (const double *)lB,
@@ -2676,7 +2690,12 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     const size_t n_traj = (size_t)K * S;
     const size_t n_part = (size_t)nbl * nb * LMAXE, n_red = (size_t)nb * LMAXE;
     const size_t n_coef = (size_t)K * nb * AND_MAX, n_rv = (size_t)K * nb * (AND_MAX + 1), n_q = (size_t)K * nb;
-    const size_t total = (grad ? 5 : 3) * n_traj + 2 * S + 3 * (size_t)BN + n_part + n_red + n_coef + n_rv + n_q;
+    // checkpoint mode: each iteration's checkpoints z(jK) (the slice store's first ck_n slots), so that the
+    // reverse sweep restores the chain's forward state instead of recomputing it (march and contour)
+    const bool keep_ck = grad && ctx->ckpt;
+    const size_t ck_bytes = (size_t)ctx->ck_n * BN * ctx->csize, n_ck = keep_ck ? (size_t)K * ck_bytes / 8 : 0;
+    const size_t total =
+        (grad ? 5 : 3) * n_traj + 2 * S + 3 * (size_t)BN + n_part + n_red + n_coef + n_rv + n_q + n_ck;
     if ((rc = ensure_loop_buffer(ctx, total * sizeof(double)))) return rc;
     double *p = ctx->loop;
     auto take = [&p](size_t n) {
@@ -2688,6 +2707,7 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     double *WTB = grad ? take(n_traj) : nullptr, *DB = grad ? take(n_traj) : nullptr;
     double *wd = take(S), *wbar = take(S), *hbar = take(BN), *hd = take(BN), *tg = take(BN);
     double *part = take(n_part), *red = take(n_red), *coefd = take(n_coef), *rvd = take(n_rv), *Qk = take(n_q);
+    char *CK = reinterpret_cast<char *>(take(n_ck));
     // pinned host side of the small transfers: sums, coefficients, Vbar/rdiag, Q
     struct Pinned {
         double *p = nullptr;
@@ -2748,6 +2768,7 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
         HIPX(ctx, enqueue_conc_scratch(ctx));
         HIPX(ctx, hipMemcpyAsync(phk, dev + 6 * BN, 8 * S, hipMemcpyDeviceToDevice, s));
         HIPX(ctx, hipMemcpyAsync(Qk + (size_t)k * nb, ctx->Q, 8 * nb, hipMemcpyDeviceToDevice, s));
+        if (keep_ck) HIPX(ctx, hipMemcpyAsync(CK + (size_t)k * ck_bytes, ctx->slots, ck_bytes, hipMemcpyDeviceToDevice, s));
         if (k == K - 1) break;
         double *ddk = DD + (size_t)k * S;
         hipLaunchKernelGGL(k_loop_resid, dim3(g1), dim3(RT), 0, s, (const double *)wtk, (const double *)phk, ddk, chiN,
@@ -2863,7 +2884,14 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
         // chain adjoint: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
         hipLaunchKernelGGL(k_loop_u, dim3(g2), dim3(RT), 0, s, (const double *)(WT + (size_t)k * S), (const double *)hd,
                            dev, BN);
-        HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0) : enqueue_conc_adjoint<double>(ctx, 0.0));
+        if (keep_ck) {  // the forward's W factors, checkpoints, Q and phi of u_k
+            HIPX(ctx, ctx->fp32 ? enqueue_prep_scratch<float>(ctx) : enqueue_prep_scratch<double>(ctx));
+            HIPX(ctx, hipMemcpyAsync(ctx->slots, CK + (size_t)k * ck_bytes, ck_bytes, hipMemcpyDeviceToDevice, s));
+            HIPX(ctx, hipMemcpyAsync(ctx->Q, Qk + (size_t)k * nb, 8 * nb, hipMemcpyDeviceToDevice, s));
+            HIPX(ctx, hipMemcpyAsync(dev + 6 * BN, PH + (size_t)k * S, 8 * S, hipMemcpyDeviceToDevice, s));
+        }
+        HIPX(ctx, ctx->fp32 ? enqueue_conc_adjoint<float>(ctx, 0.0, !keep_ck)
+                            : enqueue_conc_adjoint<double>(ctx, 0.0, !keep_ck));
         hipLaunchKernelGGL(k_loop_ubar, dim3(g2), dim3(RT), 0, s, (const double *)ubar, hbar, wtbk, BN);
         // mean removal P is symmetric: wbar(k-1) = P Wtbar(k)
         if (k > 0) center_dev(wtbk, wbar);

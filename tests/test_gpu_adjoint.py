@@ -186,6 +186,26 @@ def test_adjoint_loop_batch_matches_single(pkg, mixing):
         assert dl < 1e-12 and dg < 1e-10
 
 
+@pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
+    """sqrt(Ns) checkpointing in the loop adjoint (the reverse sweep restores
+    each iteration's checkpoints instead of re-running its forward march)
+    gives the full store's loss and gradient.  In FP32 the pair walk's
+    re-marched slices (plain steps) round differently from the forward's
+    fused step-boundary march, and unrolled Anderson amplifies that to the
+    FP32-versus-FP64 gradient level (1e-3); simple mixing keeps it at 1e-5."""
+    res = {}
+    for ck in (0, 1):
+        p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing, prec=prec, checkpoint=ck)
+        with pkg.Solver(p) as s:
+            res[ck] = s.adjoint_loop(h, w0A, w0B, iters, target)
+    dl = abs(res[1][0][0] - res[0][0][0]) / abs(res[0][0][0])
+    dg = _rel(res[1][1][0], res[0][1][0])
+    print(f"mixing={mixing} prec={prec}: checkpoint vs full rel dloss={dl:.2e} rel dgrad={dg:.2e}")
+    tol = 1e-10 if prec == 0 else (1e-5 if mixing == 0 else 1e-3)
+    assert dl < tol and dg < tol
+
+
 def test_adjoint_loop_deterministic(pkg):
     """Two calls give the same bits: every device sum runs in a fixed order."""
     p, N, h, w0A, w0B, target, iters = _loop_case(pkg, 1, anderson_reg=1e-3)
@@ -225,7 +245,7 @@ def test_adjoint_loop_long_history(pkg):
             ad = float(g[0] @ v)
             worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-12))
     print(f"m={m}, {iters} iterations: forward vs iterate {dfwd:.2e}, worst rel FD mismatch {worst:.2e}")
-    assert dfwd < 1e-8 and worst < 1e-4
+    assert dfwd < 1e-10 and worst < 1e-6
 
 
 def test_torch_autograd_wrapper(pkg):
