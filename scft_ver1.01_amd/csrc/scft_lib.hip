@@ -1312,11 +1312,14 @@ static long ck_remarch_steps(int Ns, int K)
 
 // Slice store.  Full mode: slot t holds z(t), t = 0..Ns.  Checkpoint mode
 // (sqrt(Ns) checkpointing): only z(kK) is kept (checkpoint k), plus a K-slot
-// segment buffer D (z(kK+i) recomputed from checkpoint k), a ping-pong pair P
-// and, in both modes, the adjoint ping-pong pair ZB.  Slot indices:
+// segment buffer D (z(kK+i) recomputed from checkpoint k), a ping-pong pair P,
+// a second K-slot buffer A (the segmented contour's descending range, the
+// pair walk's ascending segment) and, in both modes, the adjoint ping-pong
+// pair ZB.  Slot indices:
 //   full:  z(t) = t;                     ZB = Ns+1, Ns+2
 //   ckpt:  CK[k] = k (k < nck), D[i] = nck + i (i = 1..K-1 used),
-//          P[j] = nck + K + j, ZB[j] = nck + K + 2 + j
+//          P[j] = nck + K + j, A[i] = nck + K + 2 + i (i = 0..K-1),
+//          ZB[j] = nck + 2K + 2 + j
 // The spacing K is the one, within [K0/2, 2 K0] for K0 = ceil(sqrt(Ns+1)),
 // whose segmented contour re-marches the fewest steps with a store no larger
 // than K0's: a descending segment starts (Ns + 1) mod K steps below its range
@@ -1341,8 +1344,9 @@ static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
     if (K_out) *K_out = K;
     if (nck_out) *nck_out = nck;
     // checkpoint mode: nck checkpoints, a K-slice segment buffer (the pair
-    // walk's, and the segmented contour's ascending one), two ping-pong slices,
-    // the segmented contour's descending K-slice buffer, spare
+    // walk's descending one, and the segmented contour's ascending one), two
+    // ping-pong slices, a second K-slice buffer (the segmented contour's
+    // descending one, the pair walk's ascending one), the adjoint pair
     return ckpt ? (long)nck + 2 * K + 4 : (long)Ns + 3;
 }
 
@@ -1529,16 +1533,37 @@ static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block, typename cpl
     return hipSuccess;
 }
 
+// z(t0) = src marched n steps of the packed chain, z(t) to dst_of(t): the
+// step-boundary march where it applies, else one step at a time
+template <typename T, typename DstOf>
+static hipError_t enqueue_march_from(scft_ctx *c, const typename cplx<T>::type *src, int t0, int n, DstOf dst_of)
+{
+    using C = typename cplx<T>::type;
+    int units[3];
+    if (n <= 0) return hipSuccess;
+    if (c->p.fuse_steps && n > 1 && step_units(c, units) >= 2)
+        return enqueue_march_fused_from<T>(c, src, t0, n, dst_of, nullptr);
+    const C *cur = src;
+    for (int t = t0 + 1; t <= t0 + n; ++t) {
+        C *dst = dst_of(t);
+        hipError_t e = enqueue_step<T>(c, cur, dst, t, 0, 0);
+        if (e != hipSuccess) return e;
+        cur = dst;
+    }
+    return hipSuccess;
+}
+
 // Visits tau = 0, 1, ... and yields z(tau) (ascending) and z(Ns - tau)
-// (descending).  Full mode reads the store; checkpoint mode re-marches z(tau)
-// from checkpoint 0 (checkpoints are reused when tau hits one) and recomputes
-// the K-point segment holding Ns - tau from its checkpoint when it changes --
-// about 2 Ns extra steps for a full traversal, with O(sqrt Ns) slices held.
+// (descending).  Full mode reads the store.  Checkpoint mode keeps two
+// K-slice segment buffers: when tau reaches checkpoint j the ascending one is
+// re-marched to z(jK + 1 .. jK + K - 1), and when Ns - tau enters segment k
+// (s = kK .. kK + K - 1) the descending one is re-marched from checkpoint k --
+// about 2 Ns extra steps for a full traversal, each segment one step-boundary
+// march, with O(sqrt Ns) slices held.
 template <typename T>
 struct PairWalk {
     using C = typename cplx<T>::type;
     scft_ctx *c;
-    C *asc = nullptr;
     int seg = -1;
     explicit PairWalk(scft_ctx *cc) : c(cc) {}
 
@@ -1550,27 +1575,23 @@ struct PairWalk {
             *zu = slot_ptr<T>(c, Ns - tau);
             return hipSuccess;
         }
-        // ascending
-        if (tau % K == 0) {
-            asc = slot_ptr<T>(c, tau / K);
-        } else {
-            C *dst = slot_ptr<T>(c, nck + K + (tau & 1));
-            hipError_t e = enqueue_step<T>(c, asc, dst, tau, 0, 0);
+        // ascending: A[i] = slot nck + K + 2 + i holds z(jK + i), i = 1..K-1
+        const long a0 = nck + K + 2;
+        const int j = tau / K, ia = tau - j * K;
+        if (ia == 0) {
+            const int len = std::min(K - 1, Ns - j * K);
+            hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, j), j * K, len,
+                                                 [&](int t) { return slot_ptr<T>(c, a0 + (t - j * K)); });
             if (e != hipSuccess) return e;
-            asc = dst;
         }
-        *zt = asc;
-        // descending: segment k covers s = kK .. min(kK + K - 1, Ns)
+        *zt = (ia == 0) ? slot_ptr<T>(c, j) : slot_ptr<T>(c, a0 + ia);
+        // descending: D[i] = slot nck + i holds z(kK + i)
         const int s = Ns - tau, k = s / K;
         if (k != seg) {
             const int len = std::min(K - 1, Ns - k * K);
-            const C *prev = slot_ptr<T>(c, k);
-            for (int i = 1; i <= len; ++i) {
-                C *dst = slot_ptr<T>(c, nck + i);
-                hipError_t e = enqueue_step<T>(c, prev, dst, k * K + i, 0, 0);
-                if (e != hipSuccess) return e;
-                prev = dst;
-            }
+            hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, k), k * K, len,
+                                                 [&](int t) { return slot_ptr<T>(c, nck + (t - k * K)); });
+            if (e != hipSuccess) return e;
             seg = k;
         }
         const int i = s - k * K;
@@ -1594,18 +1615,7 @@ static hipError_t enqueue_contour_segments(scft_ctx *c, double *phiA, double *ph
     const long a0 = nck, d0 = nck + K + 2;
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     auto march = [&](const C *src, int t0, int n, auto dst_of) -> hipError_t {
-        int units[3];
-        if (n <= 0) return hipSuccess;
-        if (c->p.fuse_steps && n > 1 && step_units(c, units) >= 2)
-            return enqueue_march_fused_from<T>(c, src, t0, n, dst_of, nullptr);
-        const C *cur = src;
-        for (int t = t0 + 1; t <= t0 + n; ++t) {
-            C *dst = dst_of(t);
-            hipError_t e = enqueue_step<T>(c, cur, dst, t, 0, 0);
-            if (e != hipSuccess) return e;
-            cur = dst;
-        }
-        return hipSuccess;
+        return enqueue_march_from<T>(c, src, t0, n, dst_of);
     };
     for (int tau0 = 0; 2 * tau0 <= Ns; tau0 += K) {
         const int tau1 = std::min(tau0 + K - 1, Ns / 2);
@@ -2265,7 +2275,7 @@ static hipError_t enqueue_conc_adjoint(scft_ctx *c, double mu, bool forward = tr
     hipLaunchKernelGGL(k_adj_scalars, dim3(c->batch), dim3(RT), 0, c->stream, (const double *)c->part2, c->nblk,
                        (const double *)c->Q, mu, sc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const long zb0 = c->ckpt ? (long)c->ck_n + c->ck_K + 2 : (long)c->Ns + 1;
+    const long zb0 = c->ckpt ? (long)c->ck_n + 2 * c->ck_K + 2 : (long)c->Ns + 1;
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     PairWalk<T> walk(c);
     for (int tau = 0; tau <= c->Ns; ++tau) {
