@@ -83,10 +83,12 @@ typedef struct {
                              2 the three-level tiling (2x the threads per line; n = 32, 64,
                              128, 256) */
     int checkpoint;       /* 0: keep every q(s) slice; 1: sqrt(Ns) checkpointing -- keep
-                             z(kK) for a spacing K near sqrt(Ns+1) (the one whose segmented
-                             contour re-marches the fewest steps, with no more slices than
-                             K = ceil(sqrt(Ns+1)) would take), recompute segments (about 2x
-                             the propagator steps, O(sqrt Ns) slices of memory) */
+                             z(kK) for a spacing K near sqrt(Ns+1) (the one whose contour
+                             re-marches the fewest steps, with no more slices than
+                             K = ceil(sqrt(Ns+1)) would take); the contour is summed as the
+                             march passes each pair's second slice, re-marching only the
+                             first half's segments (about 1.4x the propagator steps,
+                             O(sqrt Ns) slices of memory) */
     int fuse_steps;       /* 1: the packed march alternates its axis order and runs each
                              step boundary (the last axis of step t, the first of t+1) as
                              one kernel; 0: every step's passes separately */

@@ -331,29 +331,31 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
     }
 }
 
-// phi contributions of the pairs (tau, Ns - tau), tau = tau0..tau1 (checkpoint
-// mode, one launch per segment): z(tau0) at alo, z(tau0 + j) at abase + j*stride;
-// z(slo) at dlo, z(slo + j) at dbase + j*stride, slo = Ns - tau1.  The segment's
-// terms are summed in FP64 registers and added to phi once.
+// Checkpoint mode: the slices of the pairs (tau, Ns - tau), tau = tau0 + k,
+// of one k_contour_pairs launch
+constexpr int CK_PAIRS = 32;
+template <typename C> struct PairSlices {
+    const C *zt[CK_PAIRS];
+    const C *zu[CK_PAIRS];
+};
+
+// phi contributions of the pairs (tau, Ns - tau), tau = tau0 .. tau0 + np - 1,
+// without the ds/Q factor (k_phi_scale applies it once Q is known): the
+// terms are summed in FP64 registers and added to phi once (init: assigned)
 template <typename T>
-__global__ void __launch_bounds__(RT) k_contour_seg(const typename cplx<T>::type *__restrict__ alo,
-                                                    const typename cplx<T>::type *__restrict__ abase,
-                                                    const typename cplx<T>::type *__restrict__ dlo,
-                                                    const typename cplx<T>::type *__restrict__ dbase, long stride,
-                                                    int tau0, int tau1, int NsA, int Ns, long N, int batch,
-                                                    const double *__restrict__ ffl, double *__restrict__ phiA,
-                                                    double *__restrict__ phiB, int init)
+__global__ void __launch_bounds__(RT) k_contour_pairs(PairSlices<typename cplx<T>::type> ps, int tau0, int np, int NsA,
+                                                      int Ns, long N, double *__restrict__ phiA,
+                                                      double *__restrict__ phiB, int init)
 {
     const int b = blockIdx.y;
     const long r = (long)blockIdx.x * RT + threadIdx.x;
     if (r >= N) return;
     const long i = b * N + r;
-    const int slo = Ns - tau1;
     double a = 0.0, c = 0.0;
-    for (int tau = tau0; tau <= tau1; ++tau) {
-        const auto zt = tau == tau0 ? alo[i] : abase[(long)(tau - tau0) * stride + i];
-        const int u = Ns - tau;
-        const auto zu = u == slo ? dlo[i] : dbase[(long)(u - slo) * stride + i];
+    for (int k = 0; k < np; ++k) {
+        const int tau = tau0 + k, u = Ns - tau;
+        const auto zt = ps.zt[k][i];
+        const auto zu = ps.zu[k][i];
         double wa, wb;
         const double p1 = (double)zt.x * (double)zu.y;
         trap_w(tau, NsA, Ns, wa, wb);
@@ -366,9 +368,20 @@ __global__ void __launch_bounds__(RT) k_contour_seg(const typename cplx<T>::type
             c += wb * p2;
         }
     }
+    phiA[i] = (init ? 0.0 : phiA[i]) + a;
+    phiB[i] = (init ? 0.0 : phiB[i]) + c;
+}
+
+// phi *= ds/Q of its instance
+__global__ void __launch_bounds__(RT) k_phi_scale(double *__restrict__ phiA, double *__restrict__ phiB,
+                                                  const double *__restrict__ ffl, long N)
+{
+    const int b = blockIdx.y;
+    const long r = (long)blockIdx.x * RT + threadIdx.x;
+    if (r >= N) return;
     const double f = ffl[b];
-    phiA[i] = (init ? 0.0 : phiA[i]) + a * f;
-    phiB[i] = (init ? 0.0 : phiB[i]) + c * f;
+    phiA[b * N + r] *= f;
+    phiB[b * N + r] *= f;
 }
 
 // partial sums of lamA*phiA + lamB*phiB -> part[(b*2)*nblk + blk]
@@ -1296,35 +1309,27 @@ extern "C" int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double
 // ---------------------------------------------------------------------------
 static unsigned pt_grid(long n) { return (unsigned)std::min<long>(4096, std::max<long>(1, (n + RT - 1) / RT)); }
 
-// Steps the segmented checkpoint contour (enqueue_contour_segments) re-marches
-// for spacing K: per segment of pairs [tau0, tau1], tau1 - tau0 steps from
-// checkpoint tau0 / K, and Ns - tau0 - kK from the checkpoint kK <= Ns - tau1.
+// Cost of a checkpoint evaluation beyond the march for spacing K
+// (enqueue_march_contour_ck): J = (Ns/2)/K segments below the last one, each
+// re-marching K - 1 steps from its checkpoint and adding two march chunks
+// (one more launch each, about half a step): J (K - 1) + J = J K.
 static long ck_remarch_steps(int Ns, int K)
 {
-    long steps = 0;
-    for (int tau0 = 0; 2 * tau0 <= Ns; tau0 += K) {
-        const int tau1 = std::min(tau0 + K - 1, Ns / 2);
-        steps += tau1 - tau0;
-        steps += (Ns - tau0) - ((Ns - tau1) / K) * K;
-    }
-    return steps;
+    return (long)((Ns / 2) / K) * K;
 }
 
 // Slice store.  Full mode: slot t holds z(t), t = 0..Ns.  Checkpoint mode
-// (sqrt(Ns) checkpointing): only z(kK) is kept (checkpoint k), plus a K-slot
-// segment buffer D (z(kK+i) recomputed from checkpoint k), a ping-pong pair P,
-// a second K-slot buffer A (the segmented contour's descending range, the
-// pair walk's ascending segment) and, in both modes, the adjoint ping-pong
-// pair ZB.  Slot indices:
+// (sqrt(Ns) checkpointing): only z(kK) is kept (checkpoint k), plus 2K + 4
+// working slots.  Slot indices:
 //   full:  z(t) = t;                     ZB = Ns+1, Ns+2
-//   ckpt:  CK[k] = k (k < nck), D[i] = nck + i (i = 1..K-1 used),
-//          P[j] = nck + K + j, A[i] = nck + K + 2 + i (i = 0..K-1),
-//          ZB[j] = nck + 2K + 2 + j
+//   ckpt:  CK[k] = k (k < nck); working slots nck .. nck + 2K + 3, used as
+//          the march-and-contour's window, segment buffers, carry and
+//          ping-pong pairs (enqueue_march_contour_ck), then by the adjoint's
+//          pair walk as D[i] = nck + i, A[i] = nck + K + 2 + i (i = 1..K-1)
+//          and ZB[j] = nck + 2K + 2 + j
 // The spacing K is the one, within [K0/2, 2 K0] for K0 = ceil(sqrt(Ns+1)),
-// whose segmented contour re-marches the fewest steps with a store no larger
-// than K0's: a descending segment starts (Ns + 1) mod K steps below its range
-// when tau0 is a multiple of K, so K0 is not the cheapest spacing in general
-// (Ns = 200: K0 = 15 re-marches 234 steps in 48 slices, K = 10 190 in 45).
+// of the lowest ck_remarch_steps with a store no larger than K0's (Ns = 200:
+// K = 15, 84 re-marched steps; Ns = 100: K = 11, 40).
 static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
 {
     int K0 = 1;
@@ -1343,10 +1348,6 @@ static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
     const int nck = Ns / K + 1;
     if (K_out) *K_out = K;
     if (nck_out) *nck_out = nck;
-    // checkpoint mode: nck checkpoints, a K-slice segment buffer (the pair
-    // walk's descending one, and the segmented contour's ascending one), two
-    // ping-pong slices, a second K-slice buffer (the segmented contour's
-    // descending one, the pair walk's ascending one), the adjoint pair
     return ckpt ? (long)nck + 2 * K + 4 : (long)Ns + 3;
 }
 
@@ -1472,7 +1473,13 @@ static typename cplx<T>::type *march_slot(scft_ctx *c, int t)
 // even steps in reverse: the last unit of step t is the first of step t+1, and
 // one kernel closes t (z(t) to its slot) and opens t+1 on the data in
 // registers.  Steps then cost nu - 1 launches instead of nu (nu >= 2 units).
-// z(t0) = src marched to z(t0 + nsteps); z(t) goes to dst_of(t)
+// z(t0) = src marched to z(t0 + nsteps); z(t) goes to dst_of(t).  The order
+// follows t itself, not t - t0, so a march split into chunks, or re-marched
+// from a checkpoint, runs its steps' units in the order the march from z(0)
+// does.  Only a chunk's first step can round differently: there a plain
+// kernel runs the first unit where the march from z(0) has a boundary kernel
+// (an xy-plane boundary kernel opens its step y then x, a plain plane kernel
+// runs x then y; FP32X rounds the carried slice to FP32).
 template <typename T, typename DstOf>
 static hipError_t enqueue_march_fused_from(scft_ctx *c, const typename cplx<T>::type *src0, int t0, int nsteps,
                                            DstOf dst_of, typename cplx<T>::type **last)
@@ -1486,7 +1493,7 @@ static hipError_t enqueue_march_fused_from(scft_ctx *c, const typename cplx<T>::
     for (int t = t0 + 1; t <= t_end; ++t) {
         dst = dst_of(t);
         for (int k = 0; k < nu; ++k) {
-            const int u = ((t - t0) & 1) ? units[k] : units[nu - 1 - k];
+            const int u = (t & 1) ? units[k] : units[nu - 1 - k];
             const bool first = k == 0, lastu = k == nu - 1;
             if (first && t > t0 + 1) continue;  // opened by step t-1's boundary kernel
             const C *src = (first && t == t0 + 1) ? src0 : work;
@@ -1534,14 +1541,15 @@ static hipError_t enqueue_march(scft_ctx *c, int nsteps, int block, typename cpl
 }
 
 // z(t0) = src marched n steps of the packed chain, z(t) to dst_of(t): the
-// step-boundary march where it applies, else one step at a time
+// step-boundary march where it applies (for n = 1 too, so that the step runs
+// its units in the order of t), else one step at a time
 template <typename T, typename DstOf>
 static hipError_t enqueue_march_from(scft_ctx *c, const typename cplx<T>::type *src, int t0, int n, DstOf dst_of)
 {
     using C = typename cplx<T>::type;
     int units[3];
     if (n <= 0) return hipSuccess;
-    if (c->p.fuse_steps && n > 1 && step_units(c, units) >= 2)
+    if (c->p.fuse_steps && step_units(c, units) >= 2)
         return enqueue_march_fused_from<T>(c, src, t0, n, dst_of, nullptr);
     const C *cur = src;
     for (int t = t0 + 1; t <= t0 + n; ++t) {
@@ -1600,43 +1608,87 @@ struct PairWalk {
     }
 };
 
-// Checkpoint-mode contour by segments of K pairs: z(tau) for tau in
-// [tau0, tau1] (re-marched from checkpoint tau0 / K into buffer A) pairs with
-// z(Ns - tau) (re-marched from the checkpoint at or below Ns - tau1 into
-// buffer D, the slices before the range through the ping-pong pair), and one
-// k_contour_seg launch adds the segment's terms to phi.
+// Checkpoint-mode march with the contour summed on the way.  Pair
+// (tau, Ns - tau), tau <= Ns/2, needs z(tau) from the first half of the march
+// and z(Ns - tau) from the second.  The march runs once, keeping every
+// checkpoint z(kK), and the pairs are taken in segments of K (tau = jK ..
+// jK + K - 1) from the last one, j = J = (Ns/2)/K, down:
+//   * segment J's slices z(JK .. Ns - JK) all lie on the march's own path
+//     (at most 2K of them): the first march chunk, z(0) -> z(Ns - JK), keeps
+//     them in a window;
+//   * each earlier segment j takes z(Ns - tau) from the next K steps of the
+//     march and z(tau) from a re-march of K - 1 steps from checkpoint j.
+// That is J (K - 1), about Ns/2, re-marched steps per evaluation.  phi is
+// summed without its ds/Q factor and scaled once Q is known from z(Ns).
+// Slots: checkpoint k at k; window z(JK + 1 + i) and the re-marched z(jK + i)
+// at nck + 1 + i and nck + i; the march's z(Ns - jK - K + 1 + i) at nck + K + i;
+// each chunk's last slice in the carry pair nck + 2K, nck + 2K + 1; the steps
+// before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3.
 template <typename T>
-static hipError_t enqueue_contour_segments(scft_ctx *c, double *phiA, double *phiB)
+static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *phiB)
 {
     using C = typename cplx<T>::type;
-    const int Ns = c->Ns, K = c->ck_K, nck = c->ck_n;
-    // buffer A: z(tau0 + j) at slot a0 + j, j = 1..K-1 (the pair walk's segment
-    // buffer); buffer D: z(slo + j) at slot d0 + j, j = 0..K-1
-    const long a0 = nck, d0 = nck + K + 2;
+    const int Ns = c->Ns, K = c->ck_K, nck = c->ck_n, half = Ns / 2, J = half / K;
+    const long carry0 = (long)nck + 2L * K, pp0 = carry0 + 2;
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
-    auto march = [&](const C *src, int t0, int n, auto dst_of) -> hipError_t {
-        return enqueue_march_from<T>(c, src, t0, n, dst_of);
+    auto ck = [&](int t) { return slot_ptr<T>(c, t / K); };
+    int launched = 0;
+    auto pairs = [&](int tau0, int tau1, auto zt_of, auto zu_of) -> hipError_t {
+        for (int t0 = tau0; t0 <= tau1; t0 += CK_PAIRS) {
+            PairSlices<C> ps{};
+            const int np = std::min(CK_PAIRS, tau1 - t0 + 1);
+            for (int k = 0; k < np; ++k) {
+                ps.zt[k] = zt_of(t0 + k);
+                ps.zu[k] = zu_of(Ns - t0 - k);
+            }
+            hipLaunchKernelGGL(k_contour_pairs<T>, gc, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N, phiA, phiB,
+                               launched++ == 0 ? 1 : 0);
+        }
+        return hipGetLastError();
     };
-    for (int tau0 = 0; 2 * tau0 <= Ns; tau0 += K) {
-        const int tau1 = std::min(tau0 + K - 1, Ns / 2);
-        // A: z(tau0 .. tau1); z(tau0) is checkpoint tau0 / K
-        const C *A0 = slot_ptr<T>(c, tau0 / K);
-        hipError_t e = march(A0, tau0, tau1 - tau0, [&](int t) { return slot_ptr<T>(c, a0 + (t - tau0)); });
-        if (e != hipSuccess) return e;
-        // D: z(s) for s in [Ns - tau1, Ns - tau0], marched from checkpoint k * K <= Ns - tau1
-        const int slo = Ns - tau1, shi = Ns - tau0, k = slo / K;
-        const C *D0 = slot_ptr<T>(c, k);
-        auto ddst = [&](int t) {
-            return t >= slo ? slot_ptr<T>(c, d0 + (t - slo)) : slot_ptr<T>(c, nck + K + (t & 1));
+    // segment J: z(0) -> z(Ns - JK) in one chunk
+    const int wlo = J * K, whi = Ns - J * K;
+    auto first_dst = [&](int t) -> C * {
+        if (t % K == 0) return ck(t);
+        if (t == whi) return slot_ptr<T>(c, carry0);
+        if (t >= wlo) return slot_ptr<T>(c, nck + 1 + (t - wlo));
+        return slot_ptr<T>(c, pp0 + (t & 1));
+    };
+    hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, 0), 0, whi, first_dst);
+    if (e != hipSuccess) return e;
+    auto first_at = [&](int t) -> const C * { return first_dst(t); };
+    if ((e = pairs(wlo, half, first_at, first_at)) != hipSuccess) return e;
+    int tcur = whi;
+    long carry = carry0;
+    for (int j = J - 1; j >= 0; --j) {
+        const int tau0 = j * K, tau1 = tau0 + K - 1, dlo = Ns - tau1, dhi = Ns - tau0;
+        const long next = carry == carry0 ? carry0 + 1 : carry0;
+        const C *src = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
+        // z(dlo .. dhi): the march's next K steps (tcur = dlo - 1)
+        auto ddst = [&](int t) -> C * {
+            if (t % K == 0) return ck(t);
+            if (t == dhi) return slot_ptr<T>(c, next);
+            return slot_ptr<T>(c, nck + K + (t - dlo));
         };
-        if ((e = march(D0, k * K, shi - k * K, ddst)) != hipSuccess) return e;
-        const C *Dlo = (slo == k * K) ? D0 : slot_ptr<T>(c, d0);
-        hipLaunchKernelGGL(k_contour_seg<T>, gc, dim3(RT), 0, c->stream, A0, (const C *)slot_ptr<T>(c, a0), Dlo,
-                           (const C *)slot_ptr<T>(c, d0), (long)c->batch * c->N, tau0, tau1, c->NsA, Ns, c->N,
-                           c->batch, (const double *)c->ffl, phiA, phiB, tau0 == 0 ? 1 : 0);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = enqueue_march_from<T>(c, src, tcur, dhi - tcur, ddst)) != hipSuccess) return e;
+        // z(tau0 + 1 .. tau1) re-marched from checkpoint j
+        auto adst = [&](int t) -> C * { return slot_ptr<T>(c, nck + (t - tau0)); };
+        if ((e = enqueue_march_from<T>(c, ck(tau0), tau0, K - 1, adst)) != hipSuccess) return e;
+        auto at = [&](int t) -> const C * { return t == tau0 ? ck(t) : adst(t); };
+        auto dt = [&](int t) -> const C * { return ddst(t); };
+        if ((e = pairs(tau0, tau1, at, dt)) != hipSuccess) return e;
+        tcur = dhi;
+        carry = next;
     }
-    return hipSuccess;
+    // Q from z(Ns) (tcur == Ns), then the ds/Q factor of phi
+    const C *zNs = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
+    const dim3 g(c->nblk, c->batch);
+    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, zNs, (const C *)c->slots, (const int2 *)nullptr, c->N,
+                       c->batch, c->nblk, c->part2);
+    hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
+                       c->ffl);
+    hipLaunchKernelGGL(k_phi_scale, gc, dim3(RT), 0, c->stream, phiA, phiB, (const double *)c->ffl, c->N);
+    return hipGetLastError();
 }
 
 // The trapezoid contour over the full slice store (k_contour): FP32 slices
@@ -1672,15 +1724,15 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
                        (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (c->ckpt) return enqueue_march_contour_ck<T>(c, phiA, phiB);  // one Ns for all instances
     C *zNs = nullptr;
     if ((e = enqueue_march<T>(c, c->Ns_max, -1, &zNs)) != hipSuccess) return e;
-    // full store: each instance's q(Ns_b) is slice Ns_b; checkpoint mode has one Ns
+    // full store: each instance's q(Ns_b) is slice Ns_b
     hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, (const C *)zNs, (const C *)c->slots,
-                       c->ckpt ? (const int2 *)nullptr : (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
+                       (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
     hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
                        c->ffl);
-    if (!c->ckpt) return launch_contour<T>(c, phiA, phiB);
-    return enqueue_contour_segments<T>(c, phiA, phiB);
+    return launch_contour<T>(c, phiA, phiB);
 }
 
 template <typename T>

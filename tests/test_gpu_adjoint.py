@@ -190,10 +190,12 @@ def test_adjoint_loop_batch_matches_single(pkg, mixing):
 def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     """sqrt(Ns) checkpointing in the loop adjoint (the reverse sweep restores
     each iteration's checkpoints instead of re-running its forward march)
-    gives the full store's loss and gradient.  In FP32 the pair walk's
-    re-marched slices (plain steps) round differently from the forward's
-    fused step-boundary march, and unrolled Anderson amplifies that to the
-    FP32-versus-FP64 gradient level (1e-3); simple mixing keeps it at 1e-5."""
+    gives the full store's loss and gradient.  In FP32 a march that starts at
+    a checkpoint (the checkpoint march's chunks, the pair walk's re-marches)
+    rounds its first step differently from the uninterrupted march (a plain
+    xy-plane kernel runs x then y where the boundary kernel runs y then x),
+    and unrolled Anderson amplifies that to the FP32-versus-FP64 gradient
+    level (1e-3); simple mixing keeps it at 1e-5."""
     res = {}
     for ck in (0, 1):
         p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing, prec=prec, checkpoint=ck)
