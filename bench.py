@@ -419,10 +419,18 @@ def main():
             "fp32x": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3), pkg.FP32X)}
         log("[bench] c3 200 iterations of the three forms from the seeded fields")
         forms["c3_128"]["convergence"] = convergence(pkg, name3, 200, (t64.tuning[0], t32.tuning[0], tx.tuning[0]))
-        log("[bench] c3 FP32 with sqrt(Ns) checkpointing")
-        tc, mc, _, _ = run_form(pkg, name3, pkg.FP32, 4, w3, checkpoint=1)
-        forms["c3_128"]["fp32_ckpt_iters_per_s"] = 1e3 / mc
-        tc.close()
+        # the 128^3 workload keeps q(s) with sqrt(Ns) checkpointing: the three forms in that mode
+        ck = {}
+        for prec, key in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32"), (pkg.FP32X, "fp32x")):
+            log(f"[bench] c3 {key} with sqrt(Ns) checkpointing")
+            tc, mc, fc, _ = run_form(pkg, name3, prec, 4, w3, checkpoint=1)
+            ck[key] = (mc, fc[0])
+            forms["c3_128"][f"{key}_ckpt_iters_per_s"] = 1e3 / mc
+            tc.close()
+        forms["c3_128"]["ratio_fp32_over_fp64_ckpt"] = ck["fp64"][0] / ck["fp32"][0]
+        forms["c3_128"]["ratio_fp32x_over_fp64_ckpt"] = ck["fp64"][0] / ck["fp32x"][0]
+        forms["c3_128"]["ckpt_trajectory_dF_fp32_vs_fp64"] = abs(ck["fp32"][1] - ck["fp64"][1])
+        forms["c3_128"]["ckpt_trajectory_dF_fp32x_vs_fp64"] = abs(ck["fp32x"][1] - ck["fp64"][1])
     if not a.no_sweep:
         inst = pkg.fields.sweep_instances(count=256, seed=4)
         pts = sum(n ** 3 for (n, L, c, f) in inst)

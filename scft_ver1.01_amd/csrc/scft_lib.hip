@@ -2280,7 +2280,7 @@ static int ensure_adjoint_scratch(scft_ctx *c)
 static int time_contour(scft_ctx *ctx, int reps, float *ms)
 {
     if (ctx->ckpt)
-        return set_err(ctx, SCFT_E_INVALID, "checkpoint mode sums the contour by segments (no single contour kernel)");
+        return set_err(ctx, SCFT_E_INVALID, "checkpoint mode sums the contour during the march (no single contour kernel)");
     HIPX(ctx, hipSetDevice(ctx->device));
     int rc = ensure_adjoint_scratch(ctx);
     if (rc) return rc;
