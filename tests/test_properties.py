@@ -9,7 +9,10 @@ NsA ds, mean(phi_B) = NsB ds on random fields.
 
 GPU: the HIP library in FP64 against the oracle on random even grids, box
 lengths, chiN, f and ds (so odd and even Ns), with checkpointing and
-step-boundary fusion each on or off: Q, phi and F after two iterations."""
+step-boundary fusion each on or off: Q, phi and F after two iterations; the
+chain adjoint against the oracle's adjoint on random cases (random lam, mu,
+checkpointing on or off); the loop gradient against central differences of
+its own forward on random small loops (simple or Anderson mixing)."""
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings
@@ -98,3 +101,59 @@ def test_gpu_matches_oracle_random_cases(pkg, orc, nx, ny, nz, lx, ly, lz, chiN,
     pa, _, hist = orc.run(op, wA.copy(), wB.copy(), 2)
     assert abs(fe[0][0] - hist[-1][0]) < 1e-11 * max(1.0, abs(hist[-1][0]))
     assert np.max(np.abs(phiA[0] - pa)) < 1e-11
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.gpu
+@GPU
+@given(nx=EVEN, ny=EVEN, nz=st.sampled_from([1, 2, 4, 6, 8]), fA=st.floats(0.2, 0.8),
+       ds=st.sampled_from([0.01, 0.015, 0.02, 0.03, 0.05]), mu=st.sampled_from([0.0, 1.0, -0.7]), ckpt=st.booleans(),
+       seed=st.integers(0, 2**20))
+def test_gpu_chain_adjoint_matches_oracle_random_cases(pkg, orc, nx, ny, nz, fA, ds, mu, ckpt, seed):
+    """L = sum(lam phi) + mu ln Q and dL/du by the library's adjoint march (its
+    checkpointed pair walk for the random Ns and spacing when ckpt) against
+    the oracle's independent FP64 adjoint: L 1e-11, gradient 1e-10."""
+    print("case", (nx, ny, nz), fA, ds, "mu", mu, "ckpt", ckpt, "seed", seed, flush=True)
+    p = pkg.default_params(nx=nx, ny=ny, nz=nz, lx=3.0, ly=3.5, lz=2.5, chiN=14.0, fA=fA, ds=ds, precision=0,
+                           checkpoint=int(ckpt))
+    N = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    uA, uB = rng.uniform(-1.5, 1.5, N), rng.uniform(-1.5, 1.5, N)
+    lA, lB = rng.standard_normal(N), rng.standard_normal(N)
+    with pkg.Solver(p) as s:
+        L, gA, gB, pa, pb, Q = s.conc_adjoint(uA, uB, lA, lB, mu)
+    op = orc.params(nx, ny, nz, 3.0, 3.5, 2.5, 14.0, fA, ds, pi=p.pi, wopt=p.wopt, wcmp=p.wcmp)
+    Lo, gAo, gBo, pao, pbo, Qo = orc.conc_adjoint(op, orc.ktable(op), uA, uB, lA, lB, mu)
+    assert abs(L[0] - Lo) <= 1e-11 * max(1.0, abs(Lo))
+    assert abs(Q[0] - Qo) / Qo < 1e-12 and _rel(pa[0], pao) < 1e-12
+    assert _rel(gA[0], gAo) < 1e-10 and _rel(gB[0], gBo) < 1e-10
+
+
+@pytest.mark.gpu
+@settings(max_examples=12, derandomize=True, deadline=None,
+          suppress_health_check=[HealthCheck.function_scoped_fixture])
+@given(n=st.sampled_from([4, 6, 8]), fA=st.floats(0.3, 0.7), ds=st.sampled_from([0.02, 0.03, 0.05]),
+       mixing=st.sampled_from([0, 1]), ckpt=st.booleans(), iters=st.integers(1, 6), seed=st.integers(0, 2**20))
+def test_gpu_loop_gradient_matches_fd_random_cases(pkg, n, fA, ds, mixing, ckpt, iters, seed):
+    """The loop adjoint's directional derivative along a random direction
+    against central differences (eps = 1e-5) of its own forward: 1e-6."""
+    print("case", n, fA, ds, "mixing", mixing, "ckpt", ckpt, "iters", iters, "seed", seed, flush=True)
+    p = pkg.default_params(nx=n, ny=n, nz=n, lx=3.5, ly=3.5, lz=3.5, chiN=16.0, fA=fA, ds=ds, precision=0,
+                           mixing=mixing, anderson_m=3, anderson_mix=0.2, anderson_reg=1e-3, checkpoint=int(ckpt))
+    N = n ** 3
+    rng = np.random.default_rng(seed)
+    w0A, w0B = pkg.fields.seeded_fields(n, n, n, 3.5, 3.5, 3.5, "lamellar", 0.7, 0.1, seed)
+    target = 0.5 + 0.3 * np.cos(2 * np.pi * pkg.fields.coords(n, n, n, 3.5, 3.5, 3.5)[0] / 3.5)
+    h = 0.05 * rng.standard_normal(N)
+    v = rng.standard_normal(N)
+    eps = 1e-5
+    with pkg.Solver(p) as s:
+        _, g, _ = s.adjoint_loop(h, w0A, w0B, iters, target)
+        lp = s.adjoint_loop(h + eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+        lm = s.adjoint_loop(h - eps * v, w0A, w0B, iters, target, grad=False)[0][0]
+    fd = (lp - lm) / (2 * eps)
+    ad = float(g[0] @ v)
+    assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-3), (fd, ad)
