@@ -308,6 +308,8 @@ def main():
                     help="with --profile and --choice: only the roofline leg (per-kernel launches, no iterations)")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
                     help="with --profile: step-boundary kernels on (1, the default) or off")
+    ap.add_argument("--ckpt", type=int, default=0, choices=[0, 1],
+                    help="with --profile: sqrt(Ns) checkpointing of the slice store off (0, the default) or on")
     ap.add_argument("--calib-copy", action="store_true",
                     help="with --profile: also copy 64 MiB -> 64 MiB three times (PMC byte calibration)")
     a = ap.parse_args()
@@ -336,10 +338,11 @@ def main():
             if a.calib_copy:
                 calibration_copy(64 << 20, 3)
             return
-        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice, fuse_steps=a.fuse)
-        rl = pass_roofline(s, prec)
-        print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "ms_per_iter": ms, "iters_per_s": 1e3 / ms,
-                          "F": fe[0], "roofline": rl}))
+        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice, fuse_steps=a.fuse, checkpoint=a.ckpt)
+        # the roofline leg times single kernels over the full slice store
+        rl = pass_roofline(s, prec) if not a.ckpt else None
+        print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "checkpoint": a.ckpt, "ms_per_iter": ms,
+                          "iters_per_s": 1e3 / ms, "F": fe[0], "roofline": rl}))
         s.close()
         if a.calib_copy:
             calibration_copy(64 << 20, 3)

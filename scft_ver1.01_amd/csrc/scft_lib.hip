@@ -340,48 +340,61 @@ template <typename C> struct PairSlices {
 };
 
 // phi contributions of the pairs (tau, Ns - tau), tau = tau0 .. tau0 + np - 1,
-// without the ds/Q factor (k_phi_scale applies it once Q is known): the
-// terms are summed in FP64 registers and added to phi once (init: assigned)
-template <typename T>
+// summed in FP64 registers and added to phi once (init: assigned); the last
+// launch of a contour (ffl non-null) multiplies the total by ds/Q.  P
+// consecutive points per thread (one 16-byte load per slice: P = 2 in FP32,
+// 1 in FP64; N % P == 0), the slices of 4 pairs in flight.
+template <typename T, int P>
 __global__ void __launch_bounds__(RT) k_contour_pairs(PairSlices<typename cplx<T>::type> ps, int tau0, int np, int NsA,
                                                       int Ns, long N, double *__restrict__ phiA,
-                                                      double *__restrict__ phiB, int init)
+                                                      double *__restrict__ phiB, int init,
+                                                      const double *__restrict__ ffl)
 {
+    using G = ZGroup<T, P>;
     const int b = blockIdx.y;
-    const long r = (long)blockIdx.x * RT + threadIdx.x;
-    if (r >= N) return;
-    const long i = b * N + r;
-    double a = 0.0, c = 0.0;
-    for (int k = 0; k < np; ++k) {
-        const int tau = tau0 + k, u = Ns - tau;
-        const auto zt = ps.zt[k][i];
-        const auto zu = ps.zu[k][i];
-        double wa, wb;
-        const double p1 = (double)zt.x * (double)zu.y;
-        trap_w(tau, NsA, Ns, wa, wb);
-        a += wa * p1;
-        c += wb * p1;
-        if (u != tau) {
-            const double p2 = (double)zu.x * (double)zt.y;
-            trap_w(u, NsA, Ns, wa, wb);
-            a += wa * p2;
-            c += wb * p2;
+    const long g = (long)blockIdx.x * RT + threadIdx.x;
+    if (g * P >= N) return;
+    const long gi = b * (N / P) + g;
+    double a[P], c[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) a[p] = c[p] = 0.0;
+    auto add = [&](int tau, const G &zt, const G &zu) {
+        const int u = Ns - tau;
+        double wa1, wb1, wa2, wb2;
+        trap_w(tau, NsA, Ns, wa1, wb1);
+        trap_w(u, NsA, Ns, wa2, wb2);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double p1 = (double)zt.z[p].x * (double)zu.z[p].y;
+            a[p] += wa1 * p1;
+            c[p] += wb1 * p1;
+            if (u != tau) {
+                const double p2 = (double)zu.z[p].x * (double)zt.z[p].y;
+                a[p] += wa2 * p2;
+                c[p] += wb2 * p2;
+            }
         }
+    };
+    int k = 0;
+    for (; k + 3 < np; k += 4) {
+        G zt[4], zu[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            zt[j] = reinterpret_cast<const G *>(ps.zt[k + j])[gi];
+            zu[j] = reinterpret_cast<const G *>(ps.zu[k + j])[gi];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) add(tau0 + k + j, zt[j], zu[j]);
     }
-    phiA[i] = (init ? 0.0 : phiA[i]) + a;
-    phiB[i] = (init ? 0.0 : phiB[i]) + c;
-}
-
-// phi *= ds/Q of its instance
-__global__ void __launch_bounds__(RT) k_phi_scale(double *__restrict__ phiA, double *__restrict__ phiB,
-                                                  const double *__restrict__ ffl, long N)
-{
-    const int b = blockIdx.y;
-    const long r = (long)blockIdx.x * RT + threadIdx.x;
-    if (r >= N) return;
-    const double f = ffl[b];
-    phiA[b * N + r] *= f;
-    phiB[b * N + r] *= f;
+    for (; k < np; ++k) add(tau0 + k, reinterpret_cast<const G *>(ps.zt[k])[gi], reinterpret_cast<const G *>(ps.zu[k])[gi]);
+    const double f = ffl ? ffl[b] : 1.0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const long i = b * N + g * P + p;
+        const double va = (init ? 0.0 : phiA[i]) + a[p], vb = (init ? 0.0 : phiB[i]) + c[p];
+        phiA[i] = ffl ? va * f : va;
+        phiB[i] = ffl ? vb * f : vb;
+    }
 }
 
 // partial sums of lamA*phiA + lamB*phiB -> part[(b*2)*nblk + blk]
@@ -1618,8 +1631,9 @@ struct PairWalk {
 //     them in a window;
 //   * each earlier segment j takes z(Ns - tau) from the next K steps of the
 //     march and z(tau) from a re-march of K - 1 steps from checkpoint j.
-// That is J (K - 1), about Ns/2, re-marched steps per evaluation.  phi is
-// summed without its ds/Q factor and scaled once Q is known from z(Ns).
+// That is J (K - 1), about Ns/2, re-marched steps per evaluation.  Q comes
+// from z(Ns) as soon as the march reaches it, before the last segment's
+// pairs, whose launch applies the ds/Q factor to the summed phi.
 // Slots: checkpoint k at k; window z(JK + 1 + i) and the re-marched z(jK + i)
 // at nck + 1 + i and nck + i; the march's z(Ns - jK - K + 1 + i) at nck + K + i;
 // each chunk's last slice in the carry pair nck + 2K, nck + 2K + 1; the steps
@@ -1633,7 +1647,10 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     auto ck = [&](int t) { return slot_ptr<T>(c, t / K); };
     int launched = 0;
-    auto pairs = [&](int tau0, int tau1, auto zt_of, auto zu_of) -> hipError_t {
+    const bool two = sizeof(C) == 8 && c->N % 2 == 0;  // FP32: two points per thread
+    const dim3 gc2((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
+    // pairs tau0..tau1; last: the contour's final launch applies ds/Q (Q already known)
+    auto pairs = [&](int tau0, int tau1, auto zt_of, auto zu_of, bool last) -> hipError_t {
         for (int t0 = tau0; t0 <= tau1; t0 += CK_PAIRS) {
             PairSlices<C> ps{};
             const int np = std::min(CK_PAIRS, tau1 - t0 + 1);
@@ -1641,9 +1658,24 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
                 ps.zt[k] = zt_of(t0 + k);
                 ps.zu[k] = zu_of(Ns - t0 - k);
             }
-            hipLaunchKernelGGL(k_contour_pairs<T>, gc, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N, phiA, phiB,
-                               launched++ == 0 ? 1 : 0);
+            const double *f = (last && t0 + np > tau1) ? (const double *)c->ffl : nullptr;
+            const int init = launched++ == 0 ? 1 : 0;
+            if (two)
+                hipLaunchKernelGGL((k_contour_pairs<T, 2>), gc2, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
+                                   phiA, phiB, init, f);
+            else
+                hipLaunchKernelGGL((k_contour_pairs<T, 1>), gc, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
+                                   phiA, phiB, init, f);
         }
+        return hipGetLastError();
+    };
+    // Q from z(Ns), once the march has reached it
+    auto q_of = [&](const C *zNs) -> hipError_t {
+        const dim3 g(c->nblk, c->batch);
+        hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, zNs, (const C *)c->slots, (const int2 *)nullptr, c->N,
+                           c->batch, c->nblk, c->part2);
+        hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
+                           c->ffl);
         return hipGetLastError();
     };
     // segment J: z(0) -> z(Ns - JK) in one chunk
@@ -1657,7 +1689,8 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, 0), 0, whi, first_dst);
     if (e != hipSuccess) return e;
     auto first_at = [&](int t) -> const C * { return first_dst(t); };
-    if ((e = pairs(wlo, half, first_at, first_at)) != hipSuccess) return e;
+    if (J == 0 && (e = q_of(first_at(Ns))) != hipSuccess) return e;  // the first chunk reached z(Ns)
+    if ((e = pairs(wlo, half, first_at, first_at, J == 0)) != hipSuccess) return e;
     int tcur = whi;
     long carry = carry0;
     for (int j = J - 1; j >= 0; --j) {
@@ -1671,23 +1704,16 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
             return slot_ptr<T>(c, nck + K + (t - dlo));
         };
         if ((e = enqueue_march_from<T>(c, src, tcur, dhi - tcur, ddst)) != hipSuccess) return e;
+        if (j == 0 && (e = q_of(ddst(Ns))) != hipSuccess) return e;  // dhi == Ns
         // z(tau0 + 1 .. tau1) re-marched from checkpoint j
         auto adst = [&](int t) -> C * { return slot_ptr<T>(c, nck + (t - tau0)); };
         if ((e = enqueue_march_from<T>(c, ck(tau0), tau0, K - 1, adst)) != hipSuccess) return e;
         auto at = [&](int t) -> const C * { return t == tau0 ? ck(t) : adst(t); };
         auto dt = [&](int t) -> const C * { return ddst(t); };
-        if ((e = pairs(tau0, tau1, at, dt)) != hipSuccess) return e;
+        if ((e = pairs(tau0, tau1, at, dt, j == 0)) != hipSuccess) return e;
         tcur = dhi;
         carry = next;
     }
-    // Q from z(Ns) (tcur == Ns), then the ds/Q factor of phi
-    const C *zNs = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
-    const dim3 g(c->nblk, c->batch);
-    hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, zNs, (const C *)c->slots, (const int2 *)nullptr, c->N,
-                       c->batch, c->nblk, c->part2);
-    hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
-                       c->ffl);
-    hipLaunchKernelGGL(k_phi_scale, gc, dim3(RT), 0, c->stream, phiA, phiB, (const double *)c->ffl, c->N);
     return hipGetLastError();
 }
 
