@@ -186,7 +186,7 @@ def test_adjoint_loop_batch_matches_single(pkg, mixing):
         assert dl < 1e-12 and dg < 1e-10
 
 
-@pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (1, 2)])
 def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     """sqrt(Ns) checkpointing in the loop adjoint (the reverse sweep restores
     each iteration's checkpoints instead of re-running its forward march)
@@ -195,7 +195,8 @@ def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     rounds its first step differently from the uninterrupted march (a plain
     xy-plane kernel runs x then y where the boundary kernel runs y then x),
     and unrolled Anderson amplifies that to the FP32-versus-FP64 gradient
-    level (1e-3); simple mixing keeps it at 1e-5."""
+    level (1e-3); simple mixing keeps it at 1e-5.  FP32X (FP32 storage,
+    FP64 transforms) is held to the FP32 bounds."""
     res = {}
     for ck in (0, 1):
         p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing, prec=prec, checkpoint=ck)
