@@ -297,8 +297,8 @@ def main():
     ap.add_argument("--no-128", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--c5-dirs", type=int, default=2,
-                    help="random directions of the c5 finite-difference check (the c5 spec: 64; about 2 s each)")
+    ap.add_argument("--c5-dirs", type=int, default=64,
+                    help="random directions of the c5 finite-difference check (the c5 spec: 64; about 0.3 s each)")
     ap.add_argument("--profile", choices=["c2_fp32", "c2_fp64", "c2_fp32x", "c3_fp32", "c3_fp64", "c3_fp32x"],
                     help="run only warmup+steps iterations of one workload/form (for rocprofv3)")
     ap.add_argument("--choice", default=None,
