@@ -97,15 +97,18 @@ def test_vtk_converter(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "fp32x"])
-def test_driver_run_matches_library(pkg, tmp_path, precision):
+@pytest.mark.parametrize("precision,keys", [("fp64", ""), ("fp32x", ""),
+                                            ("fp64", "checkpoint=1\nmixing=anderson\nanderson_m=4\n")],
+                         ids=["fp64", "fp32x", "fp64-checkpoint-anderson"])
+def test_driver_run_matches_library(pkg, tmp_path, precision, keys):
     """Full driver run on the GPU: F line every 10 iterations (the reference's
     printf format), pha files after 100, time= line; values equal the library's
-    (the FP64 form and, through precision=fp32x, the FP32X form)."""
+    (the FP64 form; through precision=fp32x the FP32X form; through checkpoint=1
+    and mixing=anderson the checkpointed store and Anderson mixing)."""
     write_ab(tmp_path / "ab.txt", n=(16, 16, 16), extra="field=lamellar\namp=1.0\nnoise=0.1\nseed=3\nexact_output=1\n"
              # FP32X: the library's default tiling, which the comparison below uses (another tiling
              # rounds differently at ~1e-7); FP64 keeps the driver's autotuning (rounding at ~1e-14)
-             f"precision={precision}\n" + ("autotune=0\n" if precision == "fp32x" else ""))
+             f"precision={precision}\n" + ("autotune=0\n" if precision == "fp32x" else "") + keys)
     r = run(["-kernal=GPU", "-step=20"], str(tmp_path))
     assert r.returncode == 0, r.stdout
     lines = [l for l in r.stdout.splitlines() if l.startswith("GPU 0 batch")]
@@ -113,8 +116,9 @@ def test_driver_run_matches_library(pkg, tmp_path, precision):
     assert lines[0].startswith("GPU 0 batch 0:    10 : ")
     assert any(l.startswith("time=") for l in r.stdout.splitlines())
     vals = [float(v) for v in lines[2].split(":")[2].replace(",", " ").split()]
+    kw = dict(checkpoint=1, mixing=pkg.MIX_ANDERSON, anderson_m=4) if keys else {}
     p = pkg.default_params(nx=16, ny=16, nz=16, lx=4.0, ly=3.0, lz=2.0, chiN=15.0, fA=0.5, ds=0.01,
-                           precision={"fp64": pkg.FP64, "fp32x": pkg.FP32X}[precision])
+                           precision={"fp64": pkg.FP64, "fp32x": pkg.FP32X}[precision], **kw)
     wA, wB = pkg.fields.seeded_fields(16, 16, 16, 4.0, 3.0, 2.0, "lamellar", 1.0, 0.1, 3)
     with pkg.Solver(p) as s:
         s.set_fields(wA, wB)
