@@ -91,16 +91,17 @@ def run_form(pkg, name, prec, steps, warmup, choice=None, **kw):
     p = workload_params(pkg, name, prec, **kw)
     wA, wB = pkg.fields.workload_fields(name)
     s = pkg.Solver(p)
+    form = {pkg.FP64: "fp64", pkg.FP32: "fp32", pkg.FP32X: "fp32x"}[prec]
     if choice is None:
         choice, tms = s.autotune(20)
-        log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} autotune choice={choice} pass_ms={tms}")
+        log(f"[bench] {name} {form} autotune choice={choice} pass_ms={tms}")
     else:
         s.set_tiling(choice)
         tms = None
-        log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} tiling set to {choice}")
+        log(f"[bench] {name} {form} tiling set to {choice}")
     s.tuning = (choice, tms)
     s.set_fields(wA, wB)
-    log(f"[bench] {name} prec={'fp32' if prec else 'fp64'} warmup {warmup}")
+    log(f"[bench] {name} {form} warmup {warmup}")
     s.iterate(warmup)
     log(f"[bench] {name} timing {steps} iterations")
     ms = s.time_iterations(steps)
