@@ -195,8 +195,9 @@ def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     rounds its first step differently from the uninterrupted march (a plain
     xy-plane kernel runs x then y where the boundary kernel runs y then x),
     and unrolled Anderson amplifies that to the FP32-versus-FP64 gradient
-    level (1e-3); simple mixing keeps it at 1e-5.  FP32X (FP32 storage,
-    FP64 transforms) is held to the FP32 bounds."""
+    level (measured 8.4e-4 for FP32 and 3.1e-4 for FP32X; bound 2e-3); simple
+    mixing keeps it at 1e-5.  FP32X (FP32 storage, FP64 transforms) is held to
+    the FP32 bounds."""
     res = {}
     for ck in (0, 1):
         p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing, prec=prec, checkpoint=ck)
@@ -205,7 +206,7 @@ def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     dl = abs(res[1][0][0] - res[0][0][0]) / abs(res[0][0][0])
     dg = _rel(res[1][1][0], res[0][1][0])
     print(f"mixing={mixing} prec={prec}: checkpoint vs full rel dloss={dl:.2e} rel dgrad={dg:.2e}")
-    tol = 1e-10 if prec == 0 else (1e-5 if mixing == 0 else 1e-3)
+    tol = 1e-10 if prec == 0 else (1e-5 if mixing == 0 else 2e-3)
     assert dl < tol and dg < tol
 
 
