@@ -61,6 +61,9 @@ def lib():
             "orc_anderson_create": (ctypes.c_void_p, [ctypes.c_long, ctypes.c_int,
                                                       ctypes.c_double, ctypes.c_double]),
             "orc_anderson_destroy": (None, [ctypes.c_void_p]),
+            "orc_anderson_set_ridge": (None, [ctypes.c_void_p, ctypes.c_double]),
+            "orc_anderson_coef": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+            "orc_anderson_history": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
             "orc_anderson_update": (ctypes.c_double, [ctypes.c_void_p, P, _dp, _dp, _dp, _dp]),
             "orc_iterate_anderson": (ctypes.c_double, [ctypes.c_void_p, P, _dp, _dp, _dp, _dp,
                                                        _dp, _dp]),
@@ -214,8 +217,27 @@ def run(p, wA, wB, iters):
 
 
 class Anderson:
-    def __init__(self, N, m, mix, wcmp=0.10):
+    def __init__(self, N, m, mix, wcmp=0.10, ridge=0.0):
+        self.N = N
         self.h = lib().orc_anderson_create(N, m, mix, wcmp)
+        if not self.h:
+            raise ValueError("Anderson history must be 0..64")
+        if ridge:
+            lib().orc_anderson_set_ridge(self.h, float(ridge))
+
+    def coef(self):
+        """least-squares coefficients c_1..c_m' of the last update"""
+        c = np.zeros(64)
+        mp = lib().orc_anderson_coef(self.h, _p(c))
+        return c[:mp].copy()
+
+    def history(self, k):
+        """entry k of the history (0 = the last update's input): (w, d), 2N each"""
+        w = np.empty(2 * self.N)
+        d = np.empty(2 * self.N)
+        if lib().orc_anderson_history(self.h, k, _p(w), _p(d)) != 0:
+            raise IndexError(k)
+        return w, d
 
     def close(self):
         if self.h:

@@ -628,20 +628,27 @@ double orc_conc_adjoint(const orc_params *p, const double *kxyzdz, const double 
 /*   w+ = w0 + sum c_i (wi - w0) + mix*(d0 + sum c_i (di - d0))              */
 /* <a,b> sums over both fields and all grid points.  m' = 0 is simple mixing */
 /* with fraction `mix`.                                                      */
+/* Optional relative ridge rho (0 by default): U + rho*tr(U)/m' * I, the    */
+/* Tikhonov form the HIP solver applies (scft_params.anderson_reg).          */
 /* ------------------------------------------------------------------------ */
 
 struct orc_anderson {
     long N;
     int m;
     double mix, wcmp;
+    double ridge;     /* relative ridge of the least squares (0: none) */
     int nhist;        /* number of stored entries */
     double *w;        /* [(m+1)][2N] ring, entry 0 = newest */
     double *d;
+    int last_mp;      /* history entries used by the last update */
+    double last_c[64];/* its least-squares coefficients c_1..c_m' */
 };
 
 orc_anderson *orc_anderson_create(long N, int m, double mix, double wcmp)
 {
-    orc_anderson *a = (orc_anderson *)calloc(1, sizeof(orc_anderson));
+    orc_anderson *a;
+    if (m < 0 || m > 64) return NULL; /* the least squares is solved in fixed 64 x 64 storage */
+    a = (orc_anderson *)calloc(1, sizeof(orc_anderson));
     a->N = N;
     a->m = m;
     a->mix = mix;
@@ -649,6 +656,27 @@ orc_anderson *orc_anderson_create(long N, int m, double mix, double wcmp)
     a->w = (double *)malloc(sizeof(double) * (size_t)(m + 1) * 2 * N);
     a->d = (double *)malloc(sizeof(double) * (size_t)(m + 1) * 2 * N);
     return a;
+}
+
+void orc_anderson_set_ridge(orc_anderson *a, double ridge)
+{
+    a->ridge = ridge;
+}
+
+int orc_anderson_coef(const orc_anderson *a, double *c)
+{
+    int i;
+    for (i = 0; i < a->last_mp; i++) c[i] = a->last_c[i];
+    return a->last_mp;
+}
+
+int orc_anderson_history(const orc_anderson *a, int k, double *w, double *d)
+{
+    const long N2 = 2 * a->N;
+    if (k < 0 || k >= a->nhist) return -1;
+    memcpy(w, a->w + (long)k * N2, sizeof(double) * (size_t)N2);
+    memcpy(d, a->d + (long)k * N2, sizeof(double) * (size_t)N2);
+    return 0;
 }
 
 void orc_anderson_destroy(orc_anderson *a)
@@ -731,9 +759,17 @@ double orc_anderson_update(orc_anderson *a, const orc_params *p, double *wA, dou
                 U[(j - 1) * mp + (i - 1)] = u;
             }
         }
+        if (a->ridge > 0.0 && mp > 0) {
+            double tr = 0.0, lam;
+            for (i = 0; i < mp; i++) tr += U[i * mp + i];
+            lam = a->ridge * tr / mp;
+            for (i = 0; i < mp; i++) U[i * mp + i] += lam;
+        }
         for (i = 0; i < mp; i++) c[i] = V[i];
         if (mp > 0 && small_solve(mp, U, c) != 0)
             for (i = 0; i < mp; i++) c[i] = 0.0;
+        a->last_mp = mp;
+        for (i = 0; i < mp; i++) a->last_c[i] = c[i];
         for (r = 0; r < N2; r++) {
             double wn = w0[r], dn = d0[r];
             for (i = 1; i <= mp; i++) {

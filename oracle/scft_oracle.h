@@ -110,6 +110,13 @@ double orc_conc_adjoint(const orc_params *p, const double *kxyzdz, const double 
 typedef struct orc_anderson orc_anderson;
 orc_anderson *orc_anderson_create(long N, int m, double mix, double wcmp);
 void orc_anderson_destroy(orc_anderson *a);
+/* relative ridge rho of the least squares, U + rho*tr(U)/m' * I (0, the default: none) */
+void orc_anderson_set_ridge(orc_anderson *a, double ridge);
+/* coefficients c_1..c_m' of the last update into c; returns m' */
+int orc_anderson_coef(const orc_anderson *a, double *c);
+/* history entry k (0 = newest input of the last update): w and residual d, 2N each
+   (A field, then B).  Returns -1 if there is no entry k. */
+int orc_anderson_history(const orc_anderson *a, int k, double *w, double *d);
 /* one Anderson update of (wA, wB) given the phi of the current w; returns the
    residual norm err = sqrt(sum(dA^2+dB^2)/sum(wA^2+wB^2)) */
 double orc_anderson_update(orc_anderson *a, const orc_params *p, double *wA, double *wB,

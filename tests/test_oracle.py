@@ -349,3 +349,95 @@ def test_anderson_reaches_simple_mixing_fixed_point(orc):
             Q, fe = and_.iterate(p, kt, a0, b0, pa, pb)
         assert fe[4] < 1e-9
         assert fe[0] == pytest.approx(hs[-1][0], abs=1e-9), chiN
+
+
+def _anderson_run(orc, m, ridge, iters=8):
+    """Anderson on an 8^3 melt; after each update yields (coefficients, history
+    entries 0..m' of w and d, the updated fields)."""
+    p = orc.params(8, 8, 8, 3.0, 3.0, 3.0, 14.0, 0.4, 0.02)
+    kt = orc.ktable(p)
+    wA, wB = _fields(0, 1.0, 21, 512)
+    and_ = orc.Anderson(512, m, 0.1, 0.10, ridge=ridge)
+    for _ in range(iters):
+        orc.minus_average(wA)
+        orc.minus_average(wB)
+        pa, pb, _ = orc.conc(p, kt, wA, wB)
+        and_.update(p, wA, wB, pa, pb)
+        c = and_.coef()
+        hist = [and_.history(k) for k in range(len(c) + 1)]
+        yield c, hist, np.concatenate([wA, wB])
+
+
+@pytest.mark.parametrize("ridge", [0.0, 1e-2])
+def test_anderson_coefficients_are_least_squares(orc, ridge):
+    """The coefficients solve min ||d0 - D c||^2 + lam ||c||^2 with D's columns
+    d0 - d_i over the stored history, lam = ridge * tr(D^T D) / m' (numpy's SVD
+    least squares on the stacked system, not the normal equations the oracle
+    solves), and the update is the affine combination sum_j alpha_j (w_j + mix
+    d_j) with alpha_0 = 1 - sum c, alpha_i = c_i."""
+    mps = []
+    for c, hist, wnew in _anderson_run(orc, 4, ridge):
+        mp = len(c)
+        mps.append(mp)
+        w0, d0 = hist[0]
+        if mp == 0:
+            assert np.allclose(wnew, w0 + 0.1 * d0, rtol=0, atol=1e-14)
+            continue
+        D = np.column_stack([d0 - hist[i][1] for i in range(1, mp + 1)])
+        lam = ridge * np.trace(D.T @ D) / mp
+        Dx = np.vstack([D, np.sqrt(lam) * np.eye(mp)]) if ridge else D
+        rhs = np.concatenate([d0, np.zeros(mp)]) if ridge else d0
+        c_ls = np.linalg.lstsq(Dx, rhs, rcond=None)[0]
+        assert np.max(np.abs(c - c_ls)) < 1e-8 * max(1.0, np.max(np.abs(c_ls))), (mp, c, c_ls)
+        alpha = np.concatenate([[1.0 - c.sum()], c])
+        w_aff = sum(alpha[j] * (hist[j][0] + 0.1 * hist[j][1]) for j in range(mp + 1))
+        assert np.max(np.abs(wnew - w_aff)) < 1e-12 * max(1.0, np.max(np.abs(wnew)))
+    assert mps == [0, 1, 2, 3, 4, 4, 4, 4]  # history grows to m, then the oldest entry drops
+
+
+def test_anderson_ridge_changes_coefficients(orc):
+    """the ridge is applied (a ridged solve is not the plain one)"""
+    c0 = [c for c, _, _ in _anderson_run(orc, 4, 0.0, 5)][-1]
+    c1 = [c for c, _, _ in _anderson_run(orc, 4, 1e-2, 5)][-1]
+    assert np.max(np.abs(c0 - c1)) > 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("npts", [2, 3])
+def test_anderson_linear_map_terminates(orc, seed, npts):
+    """Anderson without truncation is GMRES in disguise on an affine map (Walker
+    & Ni, SIAM J. Numer. Anal. 49 (2011) 1715, Thm 2.2): with phi an affine
+    function of w the residual d(w) is affine in the n = 2*npts unknowns, and
+    the iterate after n + 1 updates is the fixed point up to rounding, while
+    the first n updates are not.  Simple mixing (m = 0) is far from it."""
+    n = 2 * npts
+    rng = np.random.default_rng(seed)
+    M = 0.3 * rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.uniform(0.2, 0.5, n)
+    chiN, wcmp = 2.0, 0.10
+    p = orc.params(npts, 1, 1, 1.0, 1.0, 1.0, chiN, 0.5, 0.01)
+
+    def resid(w):
+        f = M @ w + b
+        wA, wB, pA, pB = w[:npts], w[npts:], f[:npts], f[npts:]
+        eta = (wA + wB - chiN) / 2
+        ps = 1 - pA - pB
+        return np.concatenate([chiN * pB + eta - wA - wcmp * ps, chiN * pA + eta - wB - wcmp * ps])
+
+    out = {}
+    for m in (0, n + 2):
+        and_ = orc.Anderson(npts, m, 0.5, wcmp)
+        w = np.linspace(-1, 1, n)
+        r0 = np.linalg.norm(resid(w))
+        rel = []
+        for _ in range(n + 2):
+            f = M @ w + b
+            wA, wB = w[:npts].copy(), w[npts:].copy()
+            and_.update(p, wA, wB, f[:npts].copy(), f[npts:].copy())
+            w = np.concatenate([wA, wB])
+            rel.append(np.linalg.norm(resid(w)) / r0)
+        out[m] = rel
+    acc = out[n + 2]
+    assert min(acc[:n]) > 1e-3, acc
+    assert acc[n] < 1e-8 and acc[n + 1] < 1e-13, acc
+    assert min(out[0]) > 1e-2, out[0]
