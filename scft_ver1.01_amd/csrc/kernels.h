@@ -58,8 +58,37 @@ struct PassArgs {
     int step;            // contour step (per-instance mode) or 0
 };
 
-// true if the pass kernels have an instantiation for line length n (and the
-// three-level tiling for it)
+// Radix plan of the LDS Stockham line kernel that serves even 2-3-5-smooth
+// lengths without a register tiling: n = r[0] * ... * r[nr-1], radices 4
+// first, then 2, 3, 5.  GEN_MAX bounds n (two FP64 complex lines in 64 KB of
+// LDS).
+constexpr int GEN_MAX = 2048;
+struct GenPlan {
+    int n, nr;
+    int r[16];
+};
+__host__ __device__ inline bool gen_plan(int n, GenPlan *g)
+{
+    if (n < 2 || n > GEN_MAX || (n & 1)) return false;
+    g->n = n;
+    g->nr = 0;
+    int m = n;
+    while (m % 4 == 0 && g->nr < 16) {
+        g->r[g->nr++] = 4;
+        m /= 4;
+    }
+    const int rs[3] = {2, 3, 5};
+    for (int k = 0; k < 3; ++k)
+        while (m % rs[k] == 0 && g->nr < 16) {
+            g->r[g->nr++] = rs[k];
+            m /= rs[k];
+        }
+    return m == 1;
+}
+
+// true if the pass kernels have an instantiation for line length n (the
+// register tilings, else the LDS Stockham kernel; and the three-level
+// tiling for it)
 bool pass_supported(int n);
 bool pass3_supported(int n);
 // launch one pass along `axis` (n = length of that axis) in tiling variant

@@ -12,7 +12,10 @@ bool pass_supported(int n)
     case N_: return true;
         SCFT_PASS_SIZES(SCFT_CASE)
 #undef SCFT_CASE
-    default: return false;
+    default: {
+        GenPlan g;
+        return gen_plan(n, &g);
+    }
     }
 }
 

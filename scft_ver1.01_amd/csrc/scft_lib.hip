@@ -1129,8 +1129,8 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     for (int a = 0; a < 3; ++a)
         if (n[a] > 1 && !pass_supported(n[a]))
             return set_err(nullptr, SCFT_E_UNSUPPORTED,
-                           "grid length " + std::to_string(n[a]) + " has no kernel tiling (supported: 1, 2, 4, 6, 8, "
-                           "10, 12, 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 128, 144, 160, 192, 256)");
+                           "grid length " + std::to_string(n[a]) + " has no kernel (supported: 1 and every even "
+                           "length up to " + std::to_string(GEN_MAX) + " whose prime factors are 2, 3 and 5)");
     if (p.nx == 1 && p.ny == 1 && p.nz == 1) return set_err(nullptr, SCFT_E_INVALID, "grid must have a length > 1");
 
     std::unique_ptr<scft_ctx> c(new scft_ctx());
