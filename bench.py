@@ -7,11 +7,17 @@ over Ns = 200 contour steps, Q, the trapezoid contour integral, the Anderson
 field update and the free energy) of the cylinder workload: AB diblock f=0.3,
 chiN=20, 64^3 periodic grid, box 4.2 Rg, ds=0.005, Anderson history 10, seeded
 hexagonal-star fields.  `value` is the FP32-propagator form (FP64 Q, phi, F,
-fields and least squares); the FP64 form, the FP32X form (FP32 storage and
-HBM bytes, FP64 transform arithmetic), the 128^3 gyroid workload and the
-errors of both FP32-storage forms against FP64 are reported beside it.
-Fields are resident in HBM before the timed region; timing uses HIP events
-on the solver's stream.
+fields and least squares): the median of 5 windows of K iterations (K = --steps;
+by default the K that makes a window last at least 1 s), timed with HIP events
+on the solver's stream after max(W, Anderson m) warm-up iterations.  Fields
+are resident in HBM before the timed region.  Beside it: the FP64 and FP32X
+forms, the 128^3 gyroid workload (sqrt(Ns) checkpointing, its storage mode,
+and the full store), every form also timed in windows of at least 1 s
+(median of 5), iteration HBM fractions against the nominal 8 TB/s and a
+measured copy peak, the per-kernel roofline legs, the errors of both
+FP32-storage forms against FP64 (one evaluation, along the timed trajectory
+and after 500 iterations), the 256-instance sweep, the adjoint workload and
+the CPU baseline (the oracle on the host cores).
 """
 import argparse
 import importlib.util
@@ -87,7 +93,27 @@ def iter_bytes(N, Ns, b):
     return N * (b * (6 * Ns + 2 * (Ns + 2)) + 64)
 
 
-def run_form(pkg, name, prec, steps, warmup, choice=None, **kw):
+WINDOWS = 5        # timed windows per figure; the median is reported
+MIN_WINDOW_S = 1.0  # a window of the long-window figure lasts at least this long
+
+
+def windows_ms(time_fn, k, windows=WINDOWS):
+    """elapsed ms of `windows` back-to-back windows of k iterations each"""
+    return [time_fn(k) for _ in range(windows)]
+
+
+def long_window_k(time_fn, min_s=MIN_WINDOW_S):
+    """iterations per window so that one window lasts at least min_s"""
+    ms = time_fn(5) / 5
+    return max(5, int(np.ceil(min_s * 1e3 / ms)))
+
+
+def run_form(pkg, name, prec, steps, warmup, choice=None, long_window=True, **kw):
+    """One form of a workload: autotune (or the given tiling), warm-up of
+    max(warmup, Anderson m) iterations so that the history is full, then the
+    median of WINDOWS windows of `steps` iterations (steps <= 0: windows of at
+    least MIN_WINDOW_S) and, with long_window, also the median of WINDOWS
+    windows of at least MIN_WINDOW_S each.  Returns (solver, timing, fe, phiA)."""
     p = workload_params(pkg, name, prec, **kw)
     wA, wB = pkg.fields.workload_fields(name)
     s = pkg.Solver(p)
@@ -101,13 +127,47 @@ def run_form(pkg, name, prec, steps, warmup, choice=None, **kw):
         log(f"[bench] {name} {form} tiling set to {choice}")
     s.tuning = (choice, tms)
     s.set_fields(wA, wB)
-    log(f"[bench] {name} {form} warmup {warmup}")
-    s.iterate(warmup)
-    log(f"[bench] {name} timing {steps} iterations")
-    ms = s.time_iterations(steps)
+    wrun = max(warmup, pkg.fields.WORKLOADS[name].get("anderson_m", 0))
+    log(f"[bench] {name} {form} warmup {wrun}")
+    s.iterate(wrun)
+    k = steps if steps > 0 else long_window_k(s.time_iterations)
+    log(f"[bench] {name} {form} timing {WINDOWS} windows of {k} iterations")
+    w = windows_ms(s.time_iterations, k)
+    t = {"k": k, "windows_ms": w, "ms_per_iter": float(np.median(w)) / k, "warmup_run": wrun}
+    if long_window and k * t["ms_per_iter"] < MIN_WINDOW_S * 1e3:
+        k1 = long_window_k(s.time_iterations)
+        log(f"[bench] {name} {form} timing {WINDOWS} windows of {k1} iterations (>= {MIN_WINDOW_S} s each)")
+        w1 = windows_ms(s.time_iterations, k1)
+        t.update({"k_long": k1, "windows_long_ms": w1, "ms_per_iter_long": float(np.median(w1)) / k1})
+    else:
+        t.update({"k_long": k, "windows_long_ms": w, "ms_per_iter_long": t["ms_per_iter"]})
     fe = s.free_energy()
     phiA, _, _, _ = s.get_fields()
-    return s, ms / steps, fe[0].copy(), phiA[0].copy()
+    return s, t, fe[0].copy(), phiA[0].copy()
+
+
+def trajectory_error(pkg, name, n_iter, choices):
+    """The FP64, FP32 and FP32X forms from the same seeded fields for n_iter
+    iterations: |dF|, max|dphiA|, max|dphiB| of each FP32-storage form against
+    FP64 after the last one."""
+    wA, wB = pkg.fields.workload_fields(name)
+    res = {}
+    for prec, ch in ((pkg.FP64, choices[0]), (pkg.FP32, choices[1]), (pkg.FP32X, choices[2])):
+        with pkg.Solver(workload_params(pkg, name, prec)) as s:
+            s.set_tiling(ch)
+            s.set_fields(wA, wB)
+            _, fe = s.iterate(n_iter)
+            pa, pb, _, _ = s.get_fields()
+            res[prec] = (fe[0].copy(), pa[0].copy(), pb[0].copy())
+    f64, a64, b64 = res[pkg.FP64]
+    out = {"iterations": n_iter, "F_fp64": float(f64[0]), "incomp_fp64": float(f64[4])}
+    for prec, key in ((pkg.FP32, "fp32"), (pkg.FP32X, "fp32x")):
+        f, a, b = res[prec]
+        out[f"dF_{key}"] = abs(float(f[0] - f64[0]))
+        out[f"max_dphiA_{key}"] = float(np.max(np.abs(a - a64)))
+        out[f"max_dphiB_{key}"] = float(np.max(np.abs(b - b64)))
+        out[f"incomp_{key}"] = float(f[4])
+    return out
 
 
 def single_eval_error(pkg, name, wA, wB, prec32=None):
@@ -229,13 +289,21 @@ def pass_roofline(s, prec, reps=50):
     times = {k: s.time_pass(CODES[k], reps) for k in kernels}
     share = {k: kernels[k] * times[k] for k in kernels}
     dom = max(share, key=share.get)
+    # bytes each launch moves: a pass reads and writes the field (4bN); a
+    # boundary kernel reads the field and the W factors and writes z(t) and the
+    # opened field (8bN when post = pre/N exactly, else 10bN)
     bpl = {k: (4 * b * N if not k.startswith("fused") else (8 if post_exact else 10) * b * N) for k in kernels}
-    achieved = bpl[dom] / (times[dom] * 1e-3) / 1e9
+    # algorithmic bytes of the work one launch does: a boundary kernel advances
+    # the march by one contour step (6bN: read z, read W, write z); a middle pass
+    # does a third of a step's convolutions (2bN)
+    alg = {k: (6 * b * N if k.startswith("fused") else 2 * b * N) for k in kernels}
+    achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
     # the contour kernel (once per iteration): reads every slice once (2 b N (Ns + 1)) and writes phiA, phiB
     tc = s.time_pass(CONTOUR_CODE, max(3, reps // 5))
     bc = 2 * b * N * (s.NsA + s.NsB + 1) + 16 * N
     return {"dominant": dom, "kernel": KERNEL_NAMES[dom], "ms_per_launch": times, "step_share_ms": share,
-            "bytes_per_launch": bpl, "achieved": achieved, "grid": n,
+            "bytes_per_launch": bpl, "algorithmic_bytes_per_launch": alg, "achieved": achieved,
+            "moved_GBps": bpl[dom] / (times[dom] * 1e-3) / 1e9, "grid": n,
             "contour": {"ms": tc, "bytes": bc, "GBps": bc / (tc * 1e-3) / 1e9, "frac": bc / (tc * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
 
 
@@ -259,38 +327,105 @@ def calibration_copy(nbytes, reps):
     log(f"[bench] calibration copy: {reps} x {nbytes >> 20} MiB device to device")
 
 
-def cpu_baseline(pkg, name, budget_s=12.0, max_iters=4):
+def copy_peak(nbytes=1 << 31, reps=10):
+    """Attainable HBM bandwidth on this GPU: `reps` device-to-device copies of
+    nbytes (the HIP runtime's copy kernel) between HIP events; a copy moves
+    2 * nbytes (read + write).  Returns GB/s."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipEventCreate(ctypes.byref(ev0)) or hip.hipEventCreate(ctypes.byref(ev1)):
+            raise RuntimeError("hipEventCreate failed")
+        hip.hipMemset(src, 1, nbytes)
+        best = 0.0
+        for _ in range(3):
+            hip.hipMemcpyAsync(dst, src, nbytes, 3, None)  # warm-up (hipMemcpyDeviceToDevice)
+            hip.hipEventRecord(ev0, None)
+            for _ in range(reps):
+                if hip.hipMemcpyAsync(dst, src, nbytes, 3, None):
+                    raise RuntimeError("hipMemcpyAsync failed")
+            hip.hipEventRecord(ev1, None)
+            hip.hipEventSynchronize(ev1)
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), ev0, ev1)
+            best = max(best, 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9)
+        return best
+    finally:
+        hip.hipDeviceSynchronize()
+        if ev0:
+            hip.hipEventDestroy(ev0)
+        if ev1:
+            hip.hipEventDestroy(ev1)
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
+def cpu_baseline(pkg, budget_s=10.0, max_iters=6):
+    """The oracle (the C restatement of the reference method, OpenMP) on this
+    host: the 64^3 cylinder workload with the host threads this process may
+    use (OMP_NUM_THREADS, else every core) and with one thread, and the 128^3
+    gyroid workload's time per iteration with the same threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    w = pkg.fields.WORKLOADS[name]
-    n = w["n"]
-    cores = min(16, os.cpu_count() or 1)
-    orc.set_threads(cores)
-    op = orc.params(n, n, n, w["L"], w["L"], w["L"], w["chiN"], w["fA"], w["ds"])
-    kt = orc.ktable(op)
-    wA, wB = pkg.fields.workload_fields(name)
-    N = n ** 3
-    phiA = np.empty(N)
-    phiB = np.empty(N)
-    anderson = orc.Anderson(N, w["anderson_m"], 0.1, 0.10)
-    anderson.iterate(op, kt, wA, wB, phiA, phiB)  # warm-up
-    it = 0
-    t0 = time.perf_counter()
-    while it < max_iters:
-        anderson.iterate(op, kt, wA, wB, phiA, phiB)
-        it += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": it / dt, "unit": "iters/s", "cores": orc.get_threads(), "kind": "port",
-            "sample": f"{it} Anderson SCFT iterations of {name} (64^3, Ns=200, FP64) after 1 warm-up, "
-                      f"oracle C restatement with OpenMP"}
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, int(os.environ.get("OMP_NUM_THREADS", nproc)))
+
+    def iters(name, nthreads, max_it, budget):
+        w = pkg.fields.WORKLOADS[name]
+        n = w["n"]
+        orc.set_threads(nthreads)
+        op = orc.params(n, n, n, w["L"], w["L"], w["L"], w["chiN"], w["fA"], w["ds"])
+        kt = orc.ktable(op)
+        wA, wB = pkg.fields.workload_fields(name)
+        N = n ** 3
+        phiA = np.empty(N)
+        phiB = np.empty(N)
+        anderson = orc.Anderson(N, w["anderson_m"], 0.1, 0.10, ridge=ANDERSON_RIDGE)
+        it = 0
+        t0 = time.perf_counter()
+        while it < max_it:
+            anderson.iterate(op, kt, wA, wB, phiA, phiB)
+            it += 1
+            if time.perf_counter() - t0 > budget:
+                break
+        return it, time.perf_counter() - t0
+
+    log(f"[bench] cpu baseline: 64^3 with {threads} threads")
+    iters("c2_cylinder_64", threads, 1, 0.0)  # warm-up (page-in, thread pool)
+    i2, t2 = iters("c2_cylinder_64", threads, max_iters, budget_s)
+    log("[bench] cpu baseline: 64^3 with 1 thread")
+    i1, t1 = iters("c2_cylinder_64", 1, 1, 0.0)
+    log(f"[bench] cpu baseline: 128^3 with {threads} threads")
+    i3, t3 = iters("c3_gyroid_128", threads, 1, 0.0)
+    orc.set_threads(threads)
+    return {"value": i2 / t2, "unit": "iters/s", "cores": threads, "kind": "port",
+            "sample": f"{i2} Anderson SCFT iterations (m=10, ridge {ANDERSON_RIDGE}) of the 64^3 cylinder "
+                      f"workload (Ns=200, FP64) after 1 warm-up, oracle C restatement with OpenMP on "
+                      f"{threads} threads",
+            "host_nproc": nproc, "threads_used": threads,
+            "threads_note": "OMP_NUM_THREADS (this process's CPU share) if set, else every core",
+            "c2_1thread_iters_per_s": i1 / t1, "c2_1thread_sample": f"{i1} iteration",
+            "c3_s_per_iter": t3 / i3, "c3_iters_per_s": i3 / t3,
+            "c3_sample": f"{i3} Anderson iteration (m=20) of the 128^3 gyroid workload on {threads} threads"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    # iterations per timed window (median of 5 windows); 0: windows of at least 1 s
+    ap.add_argument("--steps", type=int, default=0)
     # warm-up >= the Anderson history (m = 10 at 64^3): every timed iteration
     # runs with the full history, as a long solve does
     ap.add_argument("--warmup", type=int, default=10)
@@ -339,7 +474,9 @@ def main():
             if a.calib_copy:
                 calibration_copy(64 << 20, 3)
             return
-        s, ms, fe, _ = run_form(pkg, nm, prec, a.steps, a.warmup, choice, fuse_steps=a.fuse, checkpoint=a.ckpt)
+        s, t, fe, _ = run_form(pkg, nm, prec, max(1, a.steps), a.warmup, choice, long_window=False,
+                               fuse_steps=a.fuse, checkpoint=a.ckpt)
+        ms = t["ms_per_iter"]
         # the roofline leg times single kernels over the full slice store
         rl = pass_roofline(s, prec) if not a.ckpt else None
         print(json.dumps({"profile": a.profile, "tiling": s.tuning[0], "checkpoint": a.ckpt, "ms_per_iter": ms,
@@ -348,93 +485,80 @@ def main():
         if a.calib_copy:
             calibration_copy(64 << 20, 3)
         return
+    peak_meas = copy_peak()
+    log(f"[bench] measured copy peak {peak_meas:.0f} GB/s (nominal {HBM_PEAK_GBPS:.0f})")
+
+    def form_block(name, ckpt=0, roof=True):
+        """the three forms of one workload: rates (headline windows and >= 1 s
+        windows, median of 5), ratios, iteration HBM fractions against the
+        nominal and the measured peak, per-kernel roofline legs"""
+        res = {}
+        for prec, key in ((pkg.FP32, "fp32"), (pkg.FP64, "fp64"), (pkg.FP32X, "fp32x")):
+            sv, t, fe, phi = run_form(pkg, name, prec, a.steps, a.warmup, checkpoint=ckpt)
+            rl = pass_roofline(sv, prec, reps=50 if sv.N <= 1 << 18 else 20) if roof else None
+            res[key] = (sv, t, fe, phi, rl)
+        sv = res["fp32"][0]
+        N, Ns = sv.N, sv.NsA + sv.NsB
+        blk = {"checkpoint": ckpt, "grid": round(N ** (1 / 3)), "Ns": Ns, "windows": WINDOWS}
+        for key, (sv, t, fe, phi, rl) in res.items():
+            b = 8 if key == "fp64" else 4
+            it, itl = 1e3 / t["ms_per_iter"], 1e3 / t["ms_per_iter_long"]
+            blk[f"{key}_iters_per_s"] = it
+            blk[f"{key}_iters_per_s_long"] = itl
+            blk[f"{key}_k"] = t["k"]
+            blk[f"{key}_k_long"] = t["k_long"]
+            blk[f"{key}_windows_long_ms"] = t["windows_long_ms"]
+            blk[f"{key}_iter_hbm_frac"] = iter_bytes(N, Ns, b) * itl / 1e9 / HBM_PEAK_GBPS
+            blk[f"{key}_iter_frac_of_copy_peak"] = iter_bytes(N, Ns, b) * itl / 1e9 / peak_meas
+            blk[f"tiling_{key}"] = sv.tuning[0]
+            if rl is not None:
+                blk[f"kernels_{key}"] = rl
+        i64 = blk["fp64_iters_per_s_long"]
+        blk["ratio_fp32_over_fp64"] = blk["fp32_iters_per_s_long"] / i64
+        blk["ratio_fp32x_over_fp64"] = blk["fp32x_iters_per_s_long"] / i64
+        f64, p64 = res["fp64"][2], res["fp64"][3]
+        for key in ("fp32", "fp32x"):
+            blk[f"trajectory_dF_{key}_vs_fp64"] = abs(res[key][2][0] - f64[0])
+            blk[f"trajectory_max_dphiA_{key}_vs_fp64"] = float(np.max(np.abs(res[key][3] - p64)))
+        blk["trajectory_iterations"] = res["fp64"][1]["warmup_run"] + WINDOWS * (
+            res["fp64"][1]["k"] + (res["fp64"][1]["k_long"] if res["fp64"][1]["k_long"] != res["fp64"][1]["k"] else 0))
+        _, _, wa64, wb64 = res["fp64"][0].get_fields()
+        choices = (res["fp64"][0].tuning[0], res["fp32"][0].tuning[0], res["fp32x"][0].tuning[0])
+        head = res["fp32"][1]
+        rl32 = res["fp32"][4]
+        for key in res:
+            res[key][0].close()
+        return blk, head, rl32, (wa64[0], wb64[0]), choices
+
     name = "c2_cylinder_64"
-    s32, ms32, fe32, phi32 = run_form(pkg, name, pkg.FP32, a.steps, a.warmup)
-    s64, ms64, fe64, phi64 = run_form(pkg, name, pkg.FP64, a.steps, a.warmup)
-    sx, msx, fex, phix = run_form(pkg, name, pkg.FP32X, a.steps, a.warmup)
-    Ns = s32.NsA + s32.NsB
-    rl32 = pass_roofline(s32, 1)
-    rl64 = pass_roofline(s64, 0)
-    rlx = pass_roofline(sx, 2)
+    c2, head, rl32, fin64, ch2 = form_block(name)
     dom = rl32["dominant"]
-    N = s32.N
-    it32 = 1e3 / ms32
-    it64 = 1e3 / ms64
-    forms = {
-        "c2_64": {
-            "fp32_iters_per_s": it32, "fp64_iters_per_s": it64, "ratio_fp32_over_fp64": it32 / it64,
-            "fp32_iter_hbm_frac": iter_bytes(N, Ns, 4) * it32 / 1e9 / HBM_PEAK_GBPS,
-            "fp64_iter_hbm_frac": iter_bytes(N, Ns, 8) * it64 / 1e9 / HBM_PEAK_GBPS,
-            "trajectory_dF_fp32_vs_fp64": abs(fe32[0] - fe64[0]),
-            "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(phi32 - phi64))),
-            "F_fp64": fe64[0], "iterations_compared": a.warmup + a.steps,
-            "kernels_fp32": rl32, "kernels_fp64": rl64,
-            "fp32x_iters_per_s": 1e3 / msx, "ratio_fp32x_over_fp64": ms64 / msx,
-            "fp32x_iter_hbm_frac": iter_bytes(N, Ns, 4) * (1e3 / msx) / 1e9 / HBM_PEAK_GBPS,
-            "trajectory_dF_fp32x_vs_fp64": abs(fex[0] - fe64[0]),
-            "trajectory_max_dphiA_fp32x_vs_fp64": float(np.max(np.abs(phix - phi64))),
-            "kernels_fp32x": rlx,
-        }
-    }
-    _, _, wa64, wb64 = s64.get_fields()
-    s32.close()
-    s64.close()
-    sx.close()
-    forms["c2_64"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name, wa64[0], wb64[0])
-    forms["c2_64"]["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name, wa64[0], wb64[0], pkg.FP32X)
-    forms["c2_64"]["single_eval_seeded_fields"] = {
+    forms = {"c2_64": c2}
+    log("[bench] c2 one evaluation of the forms on the FP64 run's final fields and on the seeded fields")
+    c2["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name, *fin64)
+    c2["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name, *fin64, pkg.FP32X)
+    c2["single_eval_seeded_fields"] = {
         "fp32": single_eval_error(pkg, name, *pkg.fields.workload_fields(name)),
         "fp32x": single_eval_error(pkg, name, *pkg.fields.workload_fields(name), pkg.FP32X)}
+    log("[bench] c2 500 iterations of the three forms from the seeded fields")
+    c2["after_500_iterations"] = trajectory_error(pkg, name, 500, ch2)
     if not a.no_128:
         name3 = "c3_gyroid_128"
-        # 128^3 warm-up: the Anderson history (m = 20) fills before the timed iterations
-        w3 = pkg.fields.WORKLOADS[name3]["anderson_m"]
-        t32, m32, f32, p32 = run_form(pkg, name3, pkg.FP32, max(5, a.steps // 2), w3)
-        t64, m64, f64, p64 = run_form(pkg, name3, pkg.FP64, max(5, a.steps // 2), w3)
-        tx, mx, fx, px = run_form(pkg, name3, pkg.FP32X, max(5, a.steps // 2), w3)
-        rl3_32 = pass_roofline(t32, 1, reps=20)
-        rl3_64 = pass_roofline(t64, 0, reps=20)
-        rl3_x = pass_roofline(tx, 2, reps=20)
-        _, _, wa64, wb64 = t64.get_fields()
-        N3 = t32.N
-        Ns3 = t32.NsA + t32.NsB
-        forms["c3_128"] = {
-            "fp32_iters_per_s": 1e3 / m32, "fp64_iters_per_s": 1e3 / m64, "ratio_fp32_over_fp64": m64 / m32,
-            "fp32_iter_hbm_frac": iter_bytes(N3, Ns3, 4) * (1e3 / m32) / 1e9 / HBM_PEAK_GBPS,
-            "fp64_iter_hbm_frac": iter_bytes(N3, Ns3, 8) * (1e3 / m64) / 1e9 / HBM_PEAK_GBPS,
-            "trajectory_dF_fp32_vs_fp64": abs(f32[0] - f64[0]),
-            "trajectory_max_dphiA_fp32_vs_fp64": float(np.max(np.abs(p32 - p64))),
-            "kernels_fp32": rl3_32, "kernels_fp64": rl3_64,
-            "tiling_fp32": t32.tuning[0], "tiling_fp64": t64.tuning[0],
-            "fp32x_iters_per_s": 1e3 / mx, "ratio_fp32x_over_fp64": m64 / mx,
-            "fp32x_iter_hbm_frac": iter_bytes(N3, Ns3, 4) * (1e3 / mx) / 1e9 / HBM_PEAK_GBPS,
-            "trajectory_dF_fp32x_vs_fp64": abs(fx[0] - f64[0]),
-            "trajectory_max_dphiA_fp32x_vs_fp64": float(np.max(np.abs(px - p64))),
-            "kernels_fp32x": rl3_x, "tiling_fp32x": tx.tuning[0],
-        }
-        t32.close()
-        t64.close()
-        tx.close()
+        # the 128^3 workload keeps q(s) with sqrt(Ns) checkpointing: that mode is its headline
+        log("[bench] c3 with sqrt(Ns) checkpointing (the workload's storage mode)")
+        c3, _, _, _, _ = form_block(name3, ckpt=1, roof=False)
+        log("[bench] c3 with the full slice store")
+        c3f, _, _, fin3, ch3 = form_block(name3, ckpt=0)
+        c3["full_store"] = c3f
+        forms["c3_128"] = c3
         log("[bench] c3 one evaluation of the forms on the FP64 run's final fields and on the seeded fields")
-        forms["c3_128"]["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0])
-        forms["c3_128"]["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name3, wa64[0], wb64[0], pkg.FP32X)
-        forms["c3_128"]["single_eval_seeded_fields"] = {
+        c3["single_eval_fp32_vs_fp64"] = single_eval_error(pkg, name3, *fin3)
+        c3["single_eval_fp32x_vs_fp64"] = single_eval_error(pkg, name3, *fin3, pkg.FP32X)
+        c3["single_eval_seeded_fields"] = {
             "fp32": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3)),
             "fp32x": single_eval_error(pkg, name3, *pkg.fields.workload_fields(name3), pkg.FP32X)}
         log("[bench] c3 200 iterations of the three forms from the seeded fields")
-        forms["c3_128"]["convergence"] = convergence(pkg, name3, 200, (t64.tuning[0], t32.tuning[0], tx.tuning[0]))
-        # the 128^3 workload keeps q(s) with sqrt(Ns) checkpointing: the three forms in that mode
-        ck = {}
-        for prec, key in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32"), (pkg.FP32X, "fp32x")):
-            log(f"[bench] c3 {key} with sqrt(Ns) checkpointing")
-            tc, mc, fc, _ = run_form(pkg, name3, prec, 4, w3, checkpoint=1)
-            ck[key] = (mc, fc[0])
-            forms["c3_128"][f"{key}_ckpt_iters_per_s"] = 1e3 / mc
-            tc.close()
-        forms["c3_128"]["ratio_fp32_over_fp64_ckpt"] = ck["fp64"][0] / ck["fp32"][0]
-        forms["c3_128"]["ratio_fp32x_over_fp64_ckpt"] = ck["fp64"][0] / ck["fp32x"][0]
-        forms["c3_128"]["ckpt_trajectory_dF_fp32_vs_fp64"] = abs(ck["fp32"][1] - ck["fp64"][1])
-        forms["c3_128"]["ckpt_trajectory_dF_fp32x_vs_fp64"] = abs(ck["fp32x"][1] - ck["fp64"][1])
+        c3["convergence"] = convergence(pkg, name3, 200, ch3)
     if not a.no_sweep:
         inst = pkg.fields.sweep_instances(count=256, seed=4)
         pts = sum(n ** 3 for (n, L, c, f) in inst)
@@ -448,11 +572,12 @@ def main():
             for i, (wa, wb) in enumerate(pkg.fields.sweep_fields(inst, noise=0.1, seed=400)):
                 sw.set_fields(i, wa, wb)
             sw.iterate(1)
-            ms = sw.time_iterations(3) / 3
+            k = max(3, int(np.ceil(MIN_WINDOW_S * 1e3 / (sw.time_iterations(2) / 2))))
+            ms = float(np.median(windows_ms(sw.time_iterations, k))) / k
             nsum = sum(s_[1] + s_[2] for s_ in sw.sizes)
             b = 8 if prec == pkg.FP64 else 4
             forms["c4_sweep"][key] = {
-                "ms_per_sweep_iteration": ms, "instance_iters_per_s": len(inst) * 1e3 / ms,
+                "ms_per_sweep_iteration": ms, "k": k, "instance_iters_per_s": len(inst) * 1e3 / ms,
                 "grid_point_steps_per_s": pts * (nsum / len(inst)) * 1e3 / ms,
                 "iter_hbm_frac": sum(iter_bytes(n ** 3, s_[1] + s_[2], b) for (n, L, c, f), s_ in zip(inst, sw.sizes))
                 * 1e3 / ms / 1e9 / HBM_PEAK_GBPS,
@@ -464,33 +589,39 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, host cores)")
-        cpu = cpu_baseline(pkg, name)
+        cpu = cpu_baseline(pkg)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
         traffic = tr.get(f"c2_fp32:{dom}")
+    it32 = 1e3 / head["ms_per_iter"]
     out = {
         "metric": "SCFT iters/s at 64^3,128^3 Ns=200 (FP32 vs FP64 form), %HBM peak, dF vs FP64",
         "value": it32,
         "unit": "iters/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": head["k"],
         "warmup": a.warmup,
-        "ms_per_step": ms32,
+        "windows": WINDOWS,
+        "timing": "median of the windows of `steps` iterations each, HIP events on the solver stream; "
+                  f"{head['warmup_run']} warm-up iterations fill the Anderson history first",
+        "ms_per_step": head["ms_per_iter"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 propagator, f64 accumulation",
         "data": "synthetic: seeded SplitMix64 noise + hexagonal-star field seed",
         "config": {"workload": "c2 cylinder: AB diblock f=0.3 chiN=20, 64^3, L=4.2 Rg, ds=0.005 "
-                               "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": Ns,
+                               "(NsA=60, NsB=140), Anderson m=10", "grid": [64, 64, 64], "Ns": c2["Ns"],
                    "batch": 1},
-        "tiling": {"c2_fp32": s32.tuning[0], "c2_fp64": s64.tuning[0], "c2_fp32x": sx.tuning[0]},
         "roofline": {"bound": "hbm", "kernel": rl32["kernel"] + " (fp32, 64^3)", "achieved": rl32["achieved"],
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": rl32["achieved"] / HBM_PEAK_GBPS,
-                     "traffic": traffic, "bytes_per_launch": rl32["bytes_per_launch"][dom],
+                     "traffic": traffic, "measured_copy_peak": peak_meas,
+                     "frac_of_measured_peak": rl32["achieved"] / peak_meas,
+                     "algorithmic_bytes_per_launch": rl32["algorithmic_bytes_per_launch"][dom],
+                     "bytes_per_launch": rl32["bytes_per_launch"][dom],
                      "ms_per_launch": rl32["ms_per_launch"][dom]},
         "cpu_baseline": cpu,
         "forms": forms,
