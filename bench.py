@@ -146,6 +146,25 @@ def run_form(pkg, name, prec, steps, warmup, choice=None, long_window=True, **kw
     return s, t, fe[0].copy(), phiA[0].copy()
 
 
+def tol_overhead(pkg, name, prec, choice, n_iter=200):
+    """Host-clock rate of scft_iterate(n) with a tolerance that is never met
+    (the convergence check of every iteration, one iteration behind the GPU)
+    against the same call without one."""
+    wA, wB = pkg.fields.workload_fields(name)
+    out = {}
+    with pkg.Solver(workload_params(pkg, name, prec)) as s:
+        s.set_tiling(choice)
+        s.set_fields(wA, wB)
+        s.iterate(20)
+        for tol in (0.0, 1e-300, 0.0, 1e-300):
+            t0 = time.perf_counter()
+            s.iterate(n_iter, tol)
+            out["with_tol" if tol else "without_tol"] = n_iter / (time.perf_counter() - t0)
+    out["ratio"] = out["with_tol"] / out["without_tol"]
+    out["iterations"] = n_iter
+    return out
+
+
 def trajectory_error(pkg, name, n_iter, choices):
     """The FP64, FP32 and FP32X forms from the same seeded fields for n_iter
     iterations: |dF|, max|dphiA|, max|dphiB| of each FP32-storage form against
@@ -540,6 +559,8 @@ def main():
     c2["single_eval_seeded_fields"] = {
         "fp32": single_eval_error(pkg, name, *pkg.fields.workload_fields(name)),
         "fp32x": single_eval_error(pkg, name, *pkg.fields.workload_fields(name), pkg.FP32X)}
+    log("[bench] c2 iterate with and without a convergence tolerance")
+    c2["tol_check_rate"] = tol_overhead(pkg, name, pkg.FP32, ch2[1])
     log("[bench] c2 500 iterations of the three forms from the seeded fields")
     c2["after_500_iterations"] = trajectory_error(pkg, name, 500, ch2)
     if not a.no_128:

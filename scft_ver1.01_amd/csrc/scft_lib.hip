@@ -551,6 +551,24 @@ __global__ void __launch_bounds__(RT) k_fe_final(const double *__restrict__ fpar
     }
 }
 
+// Up to SEG_MAX contiguous FP64 segments copied by one launch (the iteration
+// state snapshots of scft_iterate with a tolerance).
+constexpr int SEG_MAX = 12;
+struct CopySegs {
+    const double *src[SEG_MAX];
+    double *dst[SEG_MAX];
+    long n[SEG_MAX];
+    int count;
+};
+__global__ void __launch_bounds__(RT) k_copy_segs(CopySegs s)
+{
+    for (int k = 0; k < s.count; ++k) {
+        const double *__restrict__ src = s.src[k];
+        double *__restrict__ dst = s.dst[k];
+        for (long i = (long)blockIdx.x * RT + threadIdx.x; i < s.n[k]; i += (long)gridDim.x * RT) dst[i] = src[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Anderson mixing (no reference counterpart).  History ring of m+1 slots of
 // (w, d) per instance, FP64.  Residual (the direction simple mixing steps in):
@@ -824,6 +842,11 @@ struct scft_ctx {
     int have_fe = 0;
     long device_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // scft_iterate(tol > 0): two device snapshots of the iteration state, pinned
+    // host slots for two iterations' free energies, and their events
+    double *tol_bk[2] = {nullptr, nullptr};
+    double *tol_fe = nullptr;
+    hipEvent_t tol_ev[2] = {nullptr, nullptr};
     std::vector<void *> allocs;
     std::string err;
 };
@@ -907,6 +930,13 @@ static void free_all(scft_ctx *c)
     c->allocs.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int k = 0; k < 2; ++k) {
+        if (c->tol_ev[k]) (void)hipEventDestroy(c->tol_ev[k]);
+        c->tol_ev[k] = nullptr;
+        c->tol_bk[k] = nullptr;  // freed with c->allocs
+    }
+    if (c->tol_fe) (void)hipHostFree(c->tol_fe);
+    c->tol_fe = nullptr;
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     c->ev0 = c->ev1 = nullptr;
     c->stream = nullptr;
@@ -1952,6 +1982,117 @@ extern "C" int scft_free_energy(scft_ctx *ctx, double *out)
     return check_fe(ctx, fe);
 }
 
+// Size in doubles of one iteration-state snapshot: the fields, phi, Q, F and,
+// with Anderson mixing, the Gram matrix, coefficients, residual norms, the
+// update counter and the one ring slot (of g and of d) the next update writes.
+static size_t state_doubles(const scft_ctx *c)
+{
+    const size_t BN = (size_t)c->batch * c->N;
+    size_t n = 4 * BN + 6 * (size_t)c->batch;
+    if (c->and_cnt) {
+        const size_t m1 = (size_t)c->p.anderson_m + 1;
+        n += (size_t)c->batch * m1 * m1 + (size_t)c->batch * AND_MAX + c->batch + 1 + 4 * BN;
+    }
+    return n;
+}
+
+// Segments between the live iteration state and snapshot bk (to_bk: save,
+// else restore); `slot` is the Anderson ring slot the next update overwrites.
+static CopySegs state_segs(scft_ctx *c, double *bk, long slot, bool to_bk)
+{
+    const size_t BN = (size_t)c->batch * c->N;
+    CopySegs sg{};
+    double *p = bk;
+    auto add = [&](double *live, size_t n) {
+        sg.src[sg.count] = to_bk ? live : p;
+        sg.dst[sg.count] = to_bk ? p : live;
+        sg.n[sg.count] = (long)n;
+        ++sg.count;
+        p += n;
+    };
+    add(c->wA, BN);
+    add(c->wB, BN);
+    add(c->phiA, BN);
+    add(c->phiB, BN);
+    add(c->Q, c->batch);
+    add(c->fe, 5 * (size_t)c->batch);
+    if (c->and_cnt) {
+        const size_t m1 = (size_t)c->p.anderson_m + 1;
+        add(c->G, (size_t)c->batch * m1 * m1);
+        add(c->coef, (size_t)c->batch * AND_MAX);
+        add(c->aerr, c->batch);
+        add(reinterpret_cast<double *>(c->and_cnt), 1);  // bits copied as they are
+        add(c->wh + (size_t)slot * 2 * BN, 2 * BN);
+        add(c->dh + (size_t)slot * 2 * BN, 2 * BN);
+    }
+    return sg;
+}
+
+// scft_iterate with a tolerance.  The host checks iteration i's inCompMax
+// while iteration i+1 already runs, so the GPU never waits for the host;
+// after every iteration one kernel saves its state to one of two snapshots,
+// and when iteration i is the first below tol, the state after i is restored
+// from its snapshot, undoing iteration i+1 entirely (Anderson history included).
+static int iterate_tol(scft_ctx *c, int n_iter, double tol, int *done, std::vector<double> &fe)
+{
+    const size_t nb = state_doubles(c);
+    for (int k = 0; k < 2; ++k) {
+        if (!c->tol_bk[k]) {
+            int rc = dalloc(c, (void **)&c->tol_bk[k], nb * sizeof(double));
+            if (rc) return rc;
+        }
+        if (!c->tol_ev[k]) HIPX(c, hipEventCreateWithFlags(&c->tol_ev[k], hipEventDisableTiming));
+    }
+    const size_t nfe = 5 * (size_t)c->batch;
+    if (!c->tol_fe) HIPX(c, hipHostMalloc((void **)&c->tol_fe, 2 * nfe * sizeof(double), hipHostMallocDefault));
+    long cnt0 = 0;
+    const long m1 = c->and_cnt ? c->p.anderson_m + 1 : 1;
+    if (c->and_cnt) {
+        HIPX(c, hipMemcpyAsync(&cnt0, c->and_cnt, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+        HIPX(c, hipStreamSynchronize(c->stream));
+    }
+    const unsigned grid = pt_grid((long)nb);
+    auto check = [&](int i, bool &below) -> int {
+        HIPX(c, hipEventSynchronize(c->tol_ev[i & 1]));
+        std::memcpy(fe.data(), c->tol_fe + (i & 1) * nfe, nfe * sizeof(double));
+        double worst = 0.0;
+        for (int b = 0; b < c->batch; ++b) worst = std::max(worst, fe[b * 5 + 4]);
+        below = worst < tol;
+        return check_fe(c, fe);
+    };
+    int it = 0;
+    for (; it < n_iter; ++it) {
+        HIPX(c, launch_iteration(c));
+        c->have_fe = 1;
+        const long slot = (cnt0 + it + 1) % m1;  // ring slot the next update writes
+        hipLaunchKernelGGL(k_copy_segs, dim3(grid), dim3(RT), 0, c->stream, state_segs(c, c->tol_bk[it & 1], slot, true));
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipMemcpyAsync(c->tol_fe + (it & 1) * nfe, c->fe, nfe * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIPX(c, hipEventRecord(c->tol_ev[it & 1], c->stream));
+        if (it >= 1) {
+            bool below = false;
+            const int rc = check(it - 1, below);
+            if (rc || below) {  // stop after iteration it-1: undo iteration it
+                const long s1 = (cnt0 + it) % m1;
+                hipLaunchKernelGGL(k_copy_segs, dim3(grid), dim3(RT), 0, c->stream,
+                                   state_segs(c, c->tol_bk[(it - 1) & 1], s1, false));
+                HIPX(c, hipGetLastError());
+                HIPX(c, hipStreamSynchronize(c->stream));
+                if (done) *done = it;
+                return rc ? rc : SCFT_OK;  // converged
+            }
+        }
+    }
+    if (it > 0) {
+        bool below = false;
+        const int rc = check(it - 1, below);
+        if (done) *done = it;
+        if (rc) return rc;
+    }
+    return SCFT_OK;
+}
+
 extern "C" int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, double *fe_out)
 {
     if (!ctx) return SCFT_E_INVALID;
@@ -1960,22 +2101,14 @@ extern "C" int scft_iterate(scft_ctx *ctx, int n_iter, double tol, int *done, do
     HIPX(ctx, hipSetDevice(ctx->device));
     std::vector<double> fe(5 * ctx->batch, 0.0);
     int it = 0;
-    for (; it < n_iter;) {
-        HIPX(ctx, launch_iteration(ctx));
-        ++it;
-        ctx->have_fe = 1;
-        if (tol > 0.0) {
-            HIPX(ctx, hipMemcpyAsync(fe.data(), ctx->fe, 8 * fe.size(), hipMemcpyDeviceToHost, ctx->stream));
-            HIPX(ctx, hipStreamSynchronize(ctx->stream));
-            int rc = check_fe(ctx, fe);
-            if (rc) {
-                if (done) *done = it;
-                return rc;
-            }
-            double worst = 0.0;
-            for (int b = 0; b < ctx->batch; ++b) worst = std::max(worst, fe[b * 5 + 4]);
-            if (worst < tol) break;
-        }
+    if (tol > 0.0) {
+        const int rc = iterate_tol(ctx, n_iter, tol, &it, fe);
+        if (done) *done = it;
+        ctx->have_conc = 0;
+        if (rc) return rc;
+    } else {
+        for (; it < n_iter; ++it) HIPX(ctx, launch_iteration(ctx));
+        if (it > 0) ctx->have_fe = 1;
     }
     if (done) *done = it;
     ctx->have_conc = 0;

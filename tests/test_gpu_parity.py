@@ -431,3 +431,42 @@ def test_time_pass_contour_leaves_state(pkg, prec):
         s.set_fields(wA, wB)
         with pytest.raises(pkg.ScftError):
             s.time_pass(16, 1)
+
+
+@pytest.mark.parametrize("mixing", [0, 1])
+def test_iterate_tol_stops_at_first_converged_iteration(pkg, mixing):
+    """scft_iterate(n, tol) checks each iteration one iteration behind the GPU
+    and undoes the extra iteration from a snapshot: it stops exactly after the
+    first iteration whose inCompMax is below tol, with the fields, phi, F and
+    the Anderson history of a solver stepped one iteration at a time (the next
+    iterations of both agree bit for bit)."""
+    p = _params(pkg, 16, 4.0, 12.0, 0.5, 0.01, mixing=mixing, anderson_m=5, anderson_mix=0.2, wopt=0.2)
+    wA, wB = pkg.fields.seeded_fields(16, 16, 16, 4.0, 4.0, 4.0, "lamellar", 0.5, 0.1, 41)
+    with pkg.Solver(p) as s0:
+        s0.set_fields(wA, wB)
+        inc = [s0.iterate(1)[1][0][4] for _ in range(30)]
+    tol = inc[24] * 1.001
+    with pkg.Solver(p) as s1, pkg.Solver(p) as s2:
+        s1.set_fields(wA, wB)
+        done, fe1 = s1.iterate(400, tol)
+        f1 = s1.get_fields()
+        s2.set_fields(wA, wB)
+        steps = 0
+        while steps < 400:
+            _, fe2 = s2.iterate(1)
+            steps += 1
+            if fe2[0][4] < tol:
+                break
+        f2 = s2.get_fields()
+        print(f"mixing={mixing}: tol={tol:.3e} done={done} stepped={steps}")
+        assert done == steps <= 25
+        assert np.array_equal(fe1, fe2)
+        for a, b in zip(f1, f2):
+            assert np.array_equal(a, b)
+        _, g1 = s1.iterate(4)
+        _, g2 = s2.iterate(4)
+        assert np.array_equal(g1, g2)
+        assert np.array_equal(s1.get_fields()[2], s2.get_fields()[2])
+        # a tolerance that is never met runs every iteration
+        d3, _ = s1.iterate(7, 1e-300)
+        assert d3 == 7
