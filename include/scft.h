@@ -239,6 +239,23 @@ int scft_sweep_autotune(scft_sweep *sw, int reps);
 /* N, NsA, NsB of instance i */
 int scft_sweep_size(const scft_sweep *sw, int i, long *N, int *NsA, int *NsB);
 
+/* scft_propagate without allocation.  No counterpart in the reference: for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  One march of block
+   `block` as scft_propagate does it (same p, block, sign and the same
+   kernels), on caller-owned DEVICE arrays: w [batch][N] (the block's field,
+   used as given), q_init [batch][N] or NULL for 1, q_out [batch][ns+1][N]
+   (ns = NsA or NsB of p), all FP64.  The W factors, the march slices, the
+   work field and the k/twiddle tables live in one static device array of the
+   library (about 21 MB on every GPU it runs on, held until the process ends);
+   batch * N may be at most 262144 (64^3) points, larger sizes return
+   SCFT_E_INVALID.  Calls on the same GPU share that array and must not
+   overlap (one host thread at a time per GPU); the call returns when q_out
+   is written.  `stream` is a hipStream_t (NULL: the null stream).  Errors:
+   scft_last_error(NULL). */
+int scft_propagate_noalloc(const scft_params *p, int device, int block, int sign, const double *w,
+                           const double *q_init, double *q_out, void *stream);
+
 /* Problem sizes: N points, NsA, NsB, device bytes allocated. */
 int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes);
 

@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "scft_conc_adjoint", "scft_adjoint_loop",
     "scft_sweep_create", "scft_sweep_destroy", "scft_sweep_last_error", "scft_sweep_set_fields",
     "scft_sweep_get_fields", "scft_sweep_iterate", "scft_sweep_time_iterations", "scft_sweep_autotune",
-    "scft_sweep_size",
+    "scft_sweep_size", "scft_propagate_noalloc",
 )
 
 
@@ -148,6 +148,8 @@ def lib():
             "scft_set_tiling": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
+            # device pointers (w, q_init, q_out) and a hipStream_t
+            "scft_propagate_noalloc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -3363,5 +3364,211 @@ extern "C" int scft_sweep_size(const scft_sweep *s, int i, long *N, int *NsA, in
     if (N) *N = c->N;
     if (NsA) *NsA = ns.x;
     if (NsB) *NsB = ns.y;
+    return SCFT_OK;
+}
+
+// ===========================================================================
+// scft_propagate_noalloc: scft_propagate on caller-owned device arrays with
+// every work buffer in one file-scope device array, so that it allocates
+// nothing (no reference counterpart; for callers that cannot allocate).
+// ===========================================================================
+namespace {
+constexpr size_t NA_MAX_POINTS = (size_t)1 << 18;  // batch * N limit (64^3)
+// tw + kf of 3 axes (with their 256-byte alignment), then one post factor per instance
+constexpr size_t NA_TABLE_BYTES = 3 * (size_t)GEN_MAX * (16 + 8) + 8192;
+// W [4][batch][N] + two march slices + the work field: 80 bytes per point in
+// FP64, plus the 256-byte alignment of the four regions
+constexpr size_t NA_SCRATCH_BYTES = NA_TABLE_BYTES + 80 * NA_MAX_POINTS + 1024;
+constexpr int NA_MAX_DEVICES = 64;
+
+// Every GPU holds its own copy of this array from the library's load until
+// the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (scratch_on).
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[NA_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+int scratch_on(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_err(nullptr, SCFT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[device] = (unsigned char *)ptr;
+    }
+    *out = g_scratch_dev[device];
+    return SCFT_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+// real parts of a slice into out[b * inst_stride + r] (q_out's [batch][ns+1][N])
+template <typename T>
+__global__ void __launch_bounds__(RT) k_take_real_to(const typename cplx<T>::type *__restrict__ slot,
+                                                     double *__restrict__ out, long n, long N, long inst_stride)
+{
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
+        out[(i / N) * inst_stride + i % N] = (double)slot[i].x;
+}
+
+template <typename T>
+static int propagate_noalloc(scft_ctx &c, unsigned char *scr, int block, int sign, const double *w,
+                             const double *q_init, double *q_out)
+{
+    using C = typename cplx<T>::type;
+    const size_t BN = (size_t)c.batch * c.N;
+    const bool dbl_tab = !c.fp32 || c.mixed;  // table entries in the arithmetic precision
+    const size_t ts = dbl_tab ? 8 : 4;
+    // tables: tw (n complex) and kf (n reals) per active axis, then the post factors
+    std::vector<unsigned char> host(NA_TABLE_BYTES, 0);
+    size_t off = 0;
+    std::vector<double> k2;
+    double fold = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        ksq_axis(c.n[a], a == 0 ? c.p.lx : (a == 1 ? c.p.ly : c.p.lz), c.p.pi, k2);
+        if (!c.active[a]) {
+            fold *= std::exp(-c.p.ds * k2[0]);
+            continue;
+        }
+        const int n = c.n[a];
+        c.tw[a] = scr + off;
+        for (int k = 0; k < n; ++k) {
+            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+            const double re = (double)cosl(ang), im = (double)-sinl(ang);
+            if (dbl_tab) {
+                std::memcpy(&host[off + 16 * (size_t)k], &re, 8);
+                std::memcpy(&host[off + 16 * (size_t)k + 8], &im, 8);
+            } else {
+                const float fr = (float)re, fi = (float)im;
+                std::memcpy(&host[off + 8 * (size_t)k], &fr, 4);
+                std::memcpy(&host[off + 8 * (size_t)k + 4], &fi, 4);
+            }
+        }
+        off = align256(off + 2 * ts * n);
+        c.kf[a] = scr + off;
+        for (int k = 0; k < n; ++k) {
+            const double v = std::exp(-c.p.ds * k2[k]);
+            if (dbl_tab) std::memcpy(&host[off + 8 * (size_t)k], &v, 8);
+            else {
+                const float f = (float)v;
+                std::memcpy(&host[off + 4 * (size_t)k], &f, 4);
+            }
+        }
+        off = align256(off + ts * n);
+    }
+    c.post = fold / (double)c.N;
+    c.post_b = (double *)(scr + off);
+    if (off + 8 * (size_t)c.batch > NA_TABLE_BYTES)
+        return set_err(nullptr, SCFT_E_INVALID, "scft_propagate_noalloc: batch above " +
+                                                    std::to_string((NA_TABLE_BYTES - off) / 8) + " instances");
+    for (int b = 0; b < c.batch; ++b) std::memcpy(&host[off + 8 * (size_t)b], &c.post, 8);
+    off = align256(off + 8 * (size_t)c.batch);
+    // the tables reach the device before any kernel of this call reads them
+    hipError_t e = hipMemcpy(scr, host.data(), off, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("table upload: ") + hipGetErrorString(e));
+    size_t p = NA_TABLE_BYTES;
+    c.wdz = scr + p;
+    p = align256(p + 4 * BN * sizeof(T));
+    C *sl[2] = {(C *)(scr + p), nullptr};
+    p = align256(p + BN * sizeof(C));
+    sl[1] = (C *)(scr + p);
+    p = align256(p + BN * sizeof(C));
+    c.work = scr + p;
+    p = align256(p + BN * sizeof(C));
+    if (p > NA_SCRATCH_BYTES) return set_err(nullptr, SCFT_E_INVALID, "no-allocation propagate: scratch overflow");
+    c.slots = sl[0];
+    T *wpre = (T *)c.wdz + (size_t)block * BN, *wpost = (T *)c.wdz + (size_t)(2 + block) * BN;
+    const int ns = block == 0 ? c.NsA : c.NsB;
+    const long stride = (long)(ns + 1) * c.N;
+    hipLaunchKernelGGL(k_wdz_one<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
+                       (const double *)c.post_b, wpre, wpost);
+    hipLaunchKernelGGL(k_seed_real<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0], q_init, (long)BN);
+    hipLaunchKernelGGL(k_take_real_to<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0],
+                       q_out + (sign == 1 ? 0 : (long)ns * c.N), (long)BN, c.N, stride);
+    e = hipGetLastError();
+    for (int t = 1; t <= ns && e == hipSuccess; ++t) {
+        e = enqueue_step<T>(&c, sl[(t - 1) & 1], sl[t & 1], 0, block, block);
+        if (e != hipSuccess) break;
+        const int s = sign == 1 ? t : ns - t;
+        hipLaunchKernelGGL(k_take_real_to<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[t & 1],
+                           q_out + (long)s * c.N, (long)BN, c.N, stride);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("no-allocation propagate: ") + hipGetErrorString(e));
+    return SCFT_OK;
+}
+
+extern "C" int scft_propagate_noalloc(const scft_params *pin, int device, int block, int sign, const double *w,
+                                      const double *q_init, double *q_out, void *stream)
+{
+    if (!pin) return set_err(nullptr, SCFT_E_INVALID, "scft_propagate_noalloc: params is NULL");
+    if (!w || !q_out) return set_err(nullptr, SCFT_E_INVALID, "scft_propagate_noalloc: w or q_out is NULL");
+    if (block != 0 && block != 1) return set_err(nullptr, SCFT_E_INVALID, "block must be 0 (A) or 1 (B)");
+    if (sign != 1 && sign != -1) return set_err(nullptr, SCFT_E_INVALID, "sign must be +1 or -1");
+    scft_params p = *pin;
+    if (p.pi == 0.0) p.pi = SCFT_REFERENCE_PI;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1 || (p.nx == 1 && p.ny == 1 && p.nz == 1))
+        return set_err(nullptr, SCFT_E_INVALID, "grid sizes must be >= 1 with a length > 1");
+    if (!(p.lx > 0 && p.ly > 0 && p.lz > 0) || !std::isfinite(p.lx) || !std::isfinite(p.ly) || !std::isfinite(p.lz))
+        return set_err(nullptr, SCFT_E_INVALID, "box lengths must be finite and > 0");
+    if (!(p.ds > 0.0) || !(p.ds < 0.5)) return set_err(nullptr, SCFT_E_INVALID, "ds must be in (0, 0.5)");
+    if (!(p.fA > 0.0 && p.fA < 1.0)) return set_err(nullptr, SCFT_E_INVALID, "fA must be in (0, 1)");
+    if (p.batch < 1) return set_err(nullptr, SCFT_E_INVALID, "batch must be >= 1");
+    if (p.precision != SCFT_FP64 && p.precision != SCFT_FP32 && p.precision != SCFT_FP32X)
+        return set_err(nullptr, SCFT_E_INVALID, "precision must be SCFT_FP64, SCFT_FP32 or SCFT_FP32X");
+    const int n[3] = {p.nx, p.ny, p.nz};
+    for (int a = 0; a < 3; ++a)
+        if (n[a] > 1 && ((n[a] & 1) || !pass_supported(n[a])))
+            return set_err(nullptr, SCFT_E_UNSUPPORTED, "grid length " + std::to_string(n[a]) + " has no kernel");
+    const size_t BN = (size_t)p.batch * p.nx * p.ny * p.nz;
+    if (BN > NA_MAX_POINTS)
+        return set_err(nullptr, SCFT_E_INVALID,
+                       "scft_propagate_noalloc: batch * N = " + std::to_string(BN) + " exceeds its scratch limit of " +
+                           std::to_string(NA_MAX_POINTS) + " points; use scft_propagate");
+    int NsA, NsB;
+    scft_contour_counts(p.fA, p.ds, &NsA, &NsB);
+    if (NsA < 1 || NsB < 1) return set_err(nullptr, SCFT_E_INVALID, "both blocks need at least one contour step");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, SCFT_E_HIP, "no HIP device");
+    if (device < 0 || device >= ndev || device >= NA_MAX_DEVICES)
+        return set_err(nullptr, SCFT_E_INVALID, "device index out of range");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    unsigned char *scr = nullptr;
+    int rc = scratch_on(device, &scr);
+    if (rc) return rc;
+    scft_ctx c;  // a view over the scratch array: owns no device memory
+    c.p = p;
+    c.device = device;
+    c.stream = (hipStream_t)stream;
+    c.own_stream = false;
+    c.NsA = NsA;
+    c.NsB = NsB;
+    c.Ns = NsA + NsB;
+    c.Ns_max = c.Ns;
+    c.N = (long)p.nx * p.ny * p.nz;
+    c.batch = p.batch;
+    c.fp32 = p.precision != SCFT_FP64;
+    c.mixed = p.precision == SCFT_FP32X;
+    c.csize = c.fp32 ? 8 : 16;
+    c.tsize = c.fp32 ? 4 : 8;
+    for (int a = 0; a < 3; ++a) {
+        c.n[a] = n[a];
+        c.active[a] = n[a] > 1;
+        if (c.active[a]) {
+            if (c.first_axis < 0) c.first_axis = a;
+            c.last_axis = a;
+        }
+    }
+    rc = c.fp32 ? propagate_noalloc<float>(c, scr, block, sign, w, q_init, q_out)
+                : propagate_noalloc<double>(c, scr, block, sign, w, q_init, q_out);
+    if (rc) return rc;
+    e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
     return SCFT_OK;
 }

@@ -85,3 +85,14 @@ def test_seeded_fields_shapes(pkg):
     assert np.max(np.abs((wA - wB) - 2 * S)) <= 0.2 + 1e-12
     w2 = pkg.fields.seeded_fields(8, 6, 4, 2.0, 3.0, 4.0, "gyroid", 1.0, 0.1, 3)
     assert np.array_equal(wA, w2[0])
+
+
+def test_propagate_noalloc_rejects_sizes_beyond_its_scratch(pkg):
+    """the no-allocation propagate refuses batch * N above its static scratch
+    (checked before any device call)"""
+    L = pkg.lib()
+    p = pkg.default_params(nx=64, ny=64, nz=64, batch=2)
+    dummy = ctypes.c_void_p(16)
+    rc = L.scft_propagate_noalloc(ctypes.byref(p), 0, 0, 1, dummy, None, dummy, None)
+    assert rc == pkg.SCFT_E_INVALID
+    assert b"scratch limit" in L.scft_last_error(None)
