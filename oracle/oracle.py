@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libscft_oracle.so")
+# SCFT_ORACLE_LIB selects another build of the same source (the sanitizer tests
+# load libscft_oracle_san.so this way)
+_LIB_PATH = os.environ.get("SCFT_ORACLE_LIB") or os.path.join(_HERE, "libscft_oracle.so")
 
 REF_PI = 3.1415926535  # struct.h:11
 

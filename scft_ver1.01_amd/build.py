@@ -80,9 +80,42 @@ def build_driver():
     return DRIVER_BIN
 
 
+SAN = os.path.join(BUILD, "san")
+SAN_LIB = os.path.join(SAN, "libscft_hip_san.so")
+SAN_DRIVER = os.path.join(SAN, "scft_san")
+SAN_VTK = os.path.join(SAN, "scft_vtk_san")
+HOST_SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized():
+    """AddressSanitizer + UndefinedBehaviorSanitizer builds of the host code,
+    for the CPU sanitizer tests (not loaded by anything else): the driver and
+    the VTK converter (g++), and the library with scft_lib.hip's host half
+    instrumented (device code unchanged; the pass objects of build_lib are
+    reused).  Needs build_lib first."""
+    os.makedirs(SAN, exist_ok=True)
+    drv = sorted(os.path.join(DRIVER_SRC, f) for f in os.listdir(DRIVER_SRC)
+                 if f.endswith(".cpp") and f != "vtk_main.cpp")
+    _run(["g++", "-O1", "-std=c++17", *HOST_SAN, f"-I{INCLUDE}", "-o", SAN_DRIVER, *drv,
+          f"-L{HERE}", "-lscft_hip", f"-Wl,-rpath,{HERE}"])
+    _run(["g++", "-O1", "-std=c++17", *HOST_SAN, f"-I{INCLUDE}", "-o", SAN_VTK,
+          os.path.join(DRIVER_SRC, "vtk_main.cpp"), os.path.join(DRIVER_SRC, "io.cpp")])
+    # host-only sanitizers on a hipcc line: each flag behind -Xarch_host
+    host_san = []
+    for f in HOST_SAN:
+        host_san += ["-Xarch_host", f]
+    obj = os.path.join(SAN, "scft_lib_san.o")
+    _run([HIPCC, *DEVICE_FLAGS, *host_san, "-c", os.path.join(CSRC, "scft_lib.hip"), "-o", obj])
+    objs = [os.path.join(BUILD, os.path.splitext(f)[0] + ".o") for f in LIB_SOURCES if f != "scft_lib.hip"]
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libasan", *host_san, "-o", SAN_LIB,
+          *objs, obj])
+    return SAN_LIB
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="library only")
+    ap.add_argument("--san", action="store_true", help="also the sanitizer builds of the host code")
     ap.add_argument("-j", type=int, default=4)
     a = ap.parse_args(argv)
     print("built", build_lib(a.j))
@@ -90,6 +123,8 @@ def main(argv=None):
         d = build_driver()
         if d:
             print("built", d)
+    if a.san:
+        print("built", build_sanitized())
 
 
 if __name__ == "__main__":
