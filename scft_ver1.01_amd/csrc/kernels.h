@@ -5,8 +5,12 @@
 //   point index   r = (k*ny + j)*nx + i          (x fastest, as init_cuda.cu:425)
 //   fields        wA, wB, phiA, phiB : double [batch][N]
 //   wdz           T [4][batch][N]       exp(-w*ds/2) of block A, B; the same / N
-//   slice store   C [Ns+1][batch][N]     z(t) = q(t) + i*qdag(Ns - t)
-//   work          C [batch][N]           the field between the three axis passes
+//   slice store   [Ns+1] complex fields  z(t) = q(t) + i*qdag(Ns - t)
+//   work          one complex field      the field between the three axis passes
+// A complex field is planar: T [2][batch][N], the real parts of all points,
+// then their imaginary parts (host code passes it as a C pointer).  The split
+// xy-plane kernel, one workgroup per part, then reads and writes whole cache
+// lines; the line kernels load a point's two parts with two coalesced loads.
 // where T is the propagator precision (double or float) and C its complex type.
 #pragma once
 

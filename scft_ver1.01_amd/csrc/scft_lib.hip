@@ -159,7 +159,6 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
                                              typename cplx<T>::type *__restrict__ slot0, int batch)
 {
     __shared__ double sh[2 * RT / 64 + 2];
-    using C = typename cplx<T>::type;
     const int b = blockIdx.y;
     const double s0 = part ? finalize_sum(part + (b * 2) * nblk, nblk, sh) : 0.0;
     const double post = post_b[b];
@@ -176,7 +175,10 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
         wdz[BN + i] = (T)eb;
         wdz[2 * BN + i] = (T)(ea * post);
         wdz[3 * BN + i] = (T)(eb * post);
-        if (slot0) slot0[i] = C{T(1), T(1)};
+        if (slot0) {  // planar: real parts, then imaginary parts
+            reinterpret_cast<T *>(slot0)[i] = T(1);
+            reinterpret_cast<T *>(slot0)[BN + i] = T(1);
+        }
     }
 }
 
@@ -193,23 +195,27 @@ __global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, lo
     }
 }
 
+// Complex fields (slices, the work field) are planar: the n = batch*N real
+// parts, then the n imaginary parts.
 // slice <- (q_init or 1) + 0i
 template <typename T>
 __global__ void __launch_bounds__(RT) k_seed_real(typename cplx<T>::type *__restrict__ slot,
                                                   const double *__restrict__ q, long n)
 {
-    using C = typename cplx<T>::type;
-    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
-        slot[i] = C{q ? (T)q[i] : T(1), T(0)};
+    T *z = reinterpret_cast<T *>(slot);
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) {
+        z[i] = q ? (T)q[i] : T(1);
+        z[n + i] = T(0);
+    }
 }
 
-// FP64 copy of the real part of a slice
+// FP64 copy of the real part of a slice (its first n entries)
 template <typename T>
 __global__ void __launch_bounds__(RT) k_take_real(const typename cplx<T>::type *__restrict__ slot,
                                                   double *__restrict__ out, long n)
 {
-    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
-        out[i] = (double)slot[i].x;
+    const T *z = reinterpret_cast<const T *>(slot);
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) out[i] = (double)z[i];
 }
 
 // partial sums of Re z(Ns) = q(Ns)   (cal_ql, cuda_aid.cu:119-168).  With
@@ -223,10 +229,11 @@ __global__ void __launch_bounds__(RT) k_qsum(const typename cplx<T>::type *__res
 {
     __shared__ double sh[2 * RT / 64 + 2];
     const int b = blockIdx.y;
-    const typename cplx<T>::type *z = slotNs;
-    if (nsab) z = slots + (long)(nsab[b].x + nsab[b].y) * batch * N;
+    const typename cplx<T>::type *zs = slotNs;
+    if (nsab) zs = slots + (long)(nsab[b].x + nsab[b].y) * batch * N;
+    const T *z = reinterpret_cast<const T *>(zs);  // the real parts come first
     double v[1] = {0.0};
-    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) v[0] += (double)z[b * N + r].x;
+    for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) v[0] += (double)z[b * N + r];
     block_sums<1>(v, sh);
     if (threadIdx.x == 0) part[(b * 2) * nblk + blockIdx.x] = v[0];
 }
@@ -252,10 +259,19 @@ __device__ __forceinline__ void trap_w(int s, int NsA, int Ns, double &wa, doubl
     wb = (s > NsA) ? ((s == Ns) ? 0.5 : 1.0) : ((s == NsA) ? 0.5 : 0.0);
 }
 
-// P consecutive complex elements of a slice, loaded as one 16-byte access
-template <typename T, int P> struct alignas(P * sizeof(typename cplx<T>::type)) ZGroup {
-    typename cplx<T>::type z[P];
+// P consecutive real (or imaginary) parts of a slice, loaded as one access
+template <typename T, int P> struct alignas(P * sizeof(T)) RGroup {
+    T v[P];
 };
+// the real and imaginary parts of P consecutive points
+template <typename T, int P> struct ZGroup {
+    RGroup<T, P> re, im;
+};
+template <typename T, int P>
+__device__ __forceinline__ ZGroup<T, P> ld_group(const RGroup<T, P> *re, long im_off, long gi)
+{
+    return ZGroup<T, P>{re[gi], re[im_off + gi]};
+}
 
 // phi by the trapezoid rule with FP64 accumulation (w_to_phi_go, cuda_aid.cu:61-118).
 // Contour point s pairs q(s) = Re z(s) with qdag(s) = Im z(Ns - s); slices s and
@@ -269,12 +285,14 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
                                                 double *__restrict__ phiA, double *__restrict__ phiB)
 {
     using G = ZGroup<T, P>;
+    using RG = RGroup<T, P>;
     const int b = blockIdx.y;
     const int NsA = nsab[b].x, Ns = nsab[b].x + nsab[b].y;
     const long g = (long)blockIdx.x * RT + threadIdx.x;
     if (g * P >= N) return;
-    const long BNg = (long)batch * N / P;
-    const G *zp = reinterpret_cast<const G *>(slots + b * N) + g;
+    const long BNg = (long)batch * N / P;  // one plane of a slice, in groups
+    const RG *zr = reinterpret_cast<const RG *>(slots) + (b * N) / P + g;
+    auto zp_at = [&](long s) { return ld_group<T, P>(zr + s * 2 * BNg, BNg, 0); };
     double accA[P], accB[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) accA[p] = accB[p] = 0.0;
@@ -284,8 +302,8 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
         G zs[K], zu[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            zs[k] = zp[(long)(s + k) * BNg];
-            zu[k] = zp[(long)(Ns - s - k) * BNg];
+            zs[k] = zp_at(s + k);
+            zu[k] = zp_at(Ns - s - k);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -295,11 +313,11 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
             trap_w(u1, NsA, Ns, wa2, wb2);
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                const double p1 = (double)zs[k].z[p].x * (double)zu[k].z[p].y;  // q(s1) qdag(s1)
+                const double p1 = (double)zs[k].re.v[p] * (double)zu[k].im.v[p];  // q(s1) qdag(s1)
                 accA[p] += wa1 * p1;
                 accB[p] += wb1 * p1;
                 if (u1 != s1) {
-                    const double p2 = (double)zu[k].z[p].x * (double)zs[k].z[p].y;  // q(u1) qdag(u1)
+                    const double p2 = (double)zu[k].re.v[p] * (double)zs[k].im.v[p];  // q(u1) qdag(u1)
                     accA[p] += wa2 * p2;
                     accB[p] += wb2 * p2;
                 }
@@ -308,17 +326,17 @@ __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__
     }
     for (; s <= half; ++s) {
         const int u1 = Ns - s;
-        const G a = zp[(long)s * BNg], c = zp[(long)u1 * BNg];
+        const G a = zp_at(s), c = zp_at(u1);
         double wa1, wb1, wa2, wb2;
         trap_w(s, NsA, Ns, wa1, wb1);
         trap_w(u1, NsA, Ns, wa2, wb2);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const double p1 = (double)a.z[p].x * (double)c.z[p].y;
+            const double p1 = (double)a.re.v[p] * (double)c.im.v[p];
             accA[p] += wa1 * p1;
             accB[p] += wb1 * p1;
             if (u1 != s) {
-                const double p2 = (double)c.z[p].x * (double)a.z[p].y;
+                const double p2 = (double)c.re.v[p] * (double)a.im.v[p];
                 accA[p] += wa2 * p2;
                 accB[p] += wb2 * p2;
             }
@@ -347,15 +365,18 @@ template <typename C> struct PairSlices {
 // 1 in FP64; N % P == 0), the slices of 4 pairs in flight.
 template <typename T, int P>
 __global__ void __launch_bounds__(RT) k_contour_pairs(PairSlices<typename cplx<T>::type> ps, int tau0, int np, int NsA,
-                                                      int Ns, long N, double *__restrict__ phiA,
+                                                      int Ns, long N, long BN, double *__restrict__ phiA,
                                                       double *__restrict__ phiB, int init,
                                                       const double *__restrict__ ffl)
 {
     using G = ZGroup<T, P>;
+    using RG = RGroup<T, P>;
     const int b = blockIdx.y;
     const long g = (long)blockIdx.x * RT + threadIdx.x;
     if (g * P >= N) return;
     const long gi = b * (N / P) + g;
+    const long BNg = BN / P;
+    auto at = [&](const typename cplx<T>::type *z) { return ld_group<T, P>(reinterpret_cast<const RG *>(z), BNg, gi); };
     double a[P], c[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) a[p] = c[p] = 0.0;
@@ -366,11 +387,11 @@ __global__ void __launch_bounds__(RT) k_contour_pairs(PairSlices<typename cplx<T
         trap_w(u, NsA, Ns, wa2, wb2);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const double p1 = (double)zt.z[p].x * (double)zu.z[p].y;
+            const double p1 = (double)zt.re.v[p] * (double)zu.im.v[p];
             a[p] += wa1 * p1;
             c[p] += wb1 * p1;
             if (u != tau) {
-                const double p2 = (double)zu.z[p].x * (double)zt.z[p].y;
+                const double p2 = (double)zu.re.v[p] * (double)zt.im.v[p];
                 a[p] += wa2 * p2;
                 c[p] += wb2 * p2;
             }
@@ -381,13 +402,13 @@ __global__ void __launch_bounds__(RT) k_contour_pairs(PairSlices<typename cplx<T
         G zt[4], zu[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            zt[j] = reinterpret_cast<const G *>(ps.zt[k + j])[gi];
-            zu[j] = reinterpret_cast<const G *>(ps.zu[k + j])[gi];
+            zt[j] = at(ps.zt[k + j]);
+            zu[j] = at(ps.zu[k + j]);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) add(tau0 + k + j, zt[j], zu[j]);
     }
-    for (; k < np; ++k) add(tau0 + k, reinterpret_cast<const G *>(ps.zt[k])[gi], reinterpret_cast<const G *>(ps.zu[k])[gi]);
+    for (; k < np; ++k) add(tau0 + k, at(ps.zt[k]), at(ps.zu[k]));
     const double f = ffl ? ffl[b] : 1.0;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -458,18 +479,21 @@ __global__ void __launch_bounds__(RT) k_adj_point(typename cplx<T>::type *__rest
     trap_w(u, NsA, Ns, au, bu);
     const double gt = ds * (at * la + bt * lb);
     const double gu = ds * (au * la + bu * lb);
-    const C z_t = zt[i], z_u = zu[i];
+    const long BN = (long)batch * N;
+    const T *ztp = reinterpret_cast<const T *>(zt), *zup = reinterpret_cast<const T *>(zu);
+    T *zbp = reinterpret_cast<T *>(zb);
+    const C z_t = C{ztp[i], ztp[BN + i]}, z_u = C{zup[i], zup[BN + i]};
     const double ztr = (double)z_t.x, zti = (double)z_t.y, zur = (double)z_u.x, zui = (double)z_u.y;
     double er = gt * ztr / q;
     double ei = gu * zti / q;
     if (tau == 0) ei += sc[b * 2 + 1] / (double)N;
     double zbr = er, zbi = ei;
     if (!init) {
-        const C p = zb[i];
-        zbr += (double)p.x;
-        zbi += (double)p.y;
+        zbr += (double)zbp[i];
+        zbi += (double)zbp[BN + i];
     }
-    zb[i] = C{(T)zbr, (T)zbi};
+    zbp[i] = (T)zbr;
+    zbp[BN + i] = (T)zbi;
     const double t1 = zbr * zui - gt * ztr * zui / q;  // qbar_dag(tau) qdag(tau) - g q qdag / Q
     const double t2 = zbi * zur;                       // qbar(u) q(u)
     double da = -ds * (at * t1 + au * t2);
@@ -1353,6 +1377,16 @@ extern "C" int scft_get_fields(scft_ctx *ctx, double *phiA, double *phiB, double
 // ---------------------------------------------------------------------------
 static unsigned pt_grid(long n) { return (unsigned)std::min<long>(4096, std::max<long>(1, (n + RT - 1) / RT)); }
 
+// points per thread of the contour kernels: 16 bytes of each plane of a slice
+// (4 FP32 or 2 FP64 values) when N allows
+template <typename T>
+static int contour_points(long N)
+{
+    const int p = 16 / (int)sizeof(T);
+    if (N % p == 0) return p;
+    return N % 2 == 0 ? 2 : 1;
+}
+
 // Cost of a checkpoint evaluation beyond the march for spacing K
 // (enqueue_march_contour_ck): J = (Ns/2)/K segments below the last one, each
 // re-marching K - 1 steps from its checkpoint and adding two march chunks
@@ -1678,8 +1712,9 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     auto ck = [&](int t) { return slot_ptr<T>(c, t / K); };
     int launched = 0;
-    const bool two = sizeof(C) == 8 && c->N % 2 == 0;  // FP32: two points per thread
-    const dim3 gc2((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
+    const int P = contour_points<T>(c->N);  // points per thread (16-byte loads of each plane)
+    const dim3 gcp((unsigned)((c->N / P + RT - 1) / RT), c->batch);
+    const long BN = (long)c->batch * c->N;
     // pairs tau0..tau1; last: the contour's final launch applies ds/Q (Q already known)
     auto pairs = [&](int tau0, int tau1, auto zt_of, auto zu_of, bool last) -> hipError_t {
         for (int t0 = tau0; t0 <= tau1; t0 += CK_PAIRS) {
@@ -1691,12 +1726,15 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
             }
             const double *f = (last && t0 + np > tau1) ? (const double *)c->ffl : nullptr;
             const int init = launched++ == 0 ? 1 : 0;
-            if (two)
-                hipLaunchKernelGGL((k_contour_pairs<T, 2>), gc2, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
-                                   phiA, phiB, init, f);
+            if (P == 4)
+                hipLaunchKernelGGL((k_contour_pairs<T, 4>), gcp, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
+                                   BN, phiA, phiB, init, f);
+            else if (P == 2)
+                hipLaunchKernelGGL((k_contour_pairs<T, 2>), gcp, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
+                                   BN, phiA, phiB, init, f);
             else
-                hipLaunchKernelGGL((k_contour_pairs<T, 1>), gc, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
-                                   phiA, phiB, init, f);
+                hipLaunchKernelGGL((k_contour_pairs<T, 1>), gcp, dim3(RT), 0, c->stream, ps, t0, np, c->NsA, Ns, c->N,
+                                   BN, phiA, phiB, init, f);
         }
         return hipGetLastError();
     };
@@ -1748,24 +1786,25 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     return hipGetLastError();
 }
 
-// The trapezoid contour over the full slice store (k_contour): FP32 slices
-// two points per thread (one 16-byte load per slice), FP64 one; 4 pairs of
-// slices in flight per thread.  (Measured at 64^3 and 128^3: 4 points per
-// thread or 2 / 8 pairs in flight move the kernel by -6 % to +25 %, under 1 %
-// of an iteration either way.)
+// The trapezoid contour over the full slice store (k_contour): P consecutive
+// points per thread, so that each plane of a slice moves in 16-byte loads (4
+// points in FP32, 2 in FP64, fewer when N is not a multiple); 4 pairs of
+// slices in flight per thread.
 template <typename T>
 static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB)
 {
     using C = typename cplx<T>::type;
-    if (sizeof(C) == 8 && c->N % 2 == 0) {
-        const dim3 gc((unsigned)((c->N / 2 + RT - 1) / RT), c->batch);
+    const int P = contour_points<T>(c->N);
+    const dim3 gc((unsigned)((c->N / P + RT - 1) / RT), c->batch);
+    if (P == 4)
+        hipLaunchKernelGGL((k_contour<T, 4>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
+                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+    else if (P == 2)
         hipLaunchKernelGGL((k_contour<T, 2>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
                            (const int2 *)c->nsab, c->ffl, phiA, phiB);
-    } else {
-        const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
+    else
         hipLaunchKernelGGL((k_contour<T, 1>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
                            (const int2 *)c->nsab, c->ffl, phiA, phiB);
-    }
     return hipGetLastError();
 }
 
@@ -3412,8 +3451,9 @@ template <typename T>
 __global__ void __launch_bounds__(RT) k_take_real_to(const typename cplx<T>::type *__restrict__ slot,
                                                      double *__restrict__ out, long n, long N, long inst_stride)
 {
+    const T *z = reinterpret_cast<const T *>(slot);  // planar: the real parts come first
     for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT)
-        out[(i / N) * inst_stride + i % N] = (double)slot[i].x;
+        out[(i / N) * inst_stride + i % N] = (double)z[i];
 }
 
 template <typename T>
