@@ -165,6 +165,42 @@ def tol_overhead(pkg, name, prec, choice, n_iter=200):
     return out
 
 
+def precision_schedule(pkg, name, n_iter, switch_at, choices, ckpt):
+    """A solve that runs its first switch_at iterations in the FP32 form and the
+    rest in FP32X (scft_set_precision on the same context: fields, phi and the
+    Anderson history carry over), against the FP64 form over the same n_iter
+    iterations from the seeded fields: |dF|, max|dphiA| and the incompressibility
+    error at the end, and the host-clock rate of each whole solve.  choices:
+    tilings (fp64, fp32, fp32x)."""
+    wA, wB = pkg.fields.workload_fields(name)
+    res = {}
+    with pkg.Solver(workload_params(pkg, name, pkg.FP64, checkpoint=ckpt)) as s:
+        s.set_tiling(choices[0])
+        s.set_fields(wA, wB)
+        s.iterate(1)  # graph capture outside the timed solve
+        s.set_fields(wA, wB)
+        t0 = time.perf_counter()
+        _, fe = s.iterate(n_iter)
+        res["fp64"] = (time.perf_counter() - t0, fe[0].copy(), s.get_fields()[0][0].copy())
+    with pkg.Solver(workload_params(pkg, name, pkg.FP32, checkpoint=ckpt)) as s:
+        s.set_tiling(choices[1])
+        s.set_fields(wA, wB)
+        s.iterate(1)
+        s.set_fields(wA, wB)
+        t0 = time.perf_counter()
+        s.iterate(switch_at)
+        s.set_precision(pkg.FP32X)
+        s.set_tiling(choices[2])
+        _, fe = s.iterate(n_iter - switch_at)
+        res["mixed"] = (time.perf_counter() - t0, fe[0].copy(), s.get_fields()[0][0].copy())
+    (t64, f64, p64), (tm, fm, pm) = res["fp64"], res["mixed"]
+    return {"iterations": n_iter, "switch_at": switch_at, "checkpoint": ckpt,
+            "schedule": f"FP32 for {switch_at} iterations, then FP32X",
+            "dF_vs_fp64": abs(float(fm[0] - f64[0])), "max_dphiA_vs_fp64": float(np.max(np.abs(pm - p64))),
+            "incomp_fp64": float(f64[4]), "incomp_schedule": float(fm[4]), "F_fp64": float(f64[0]),
+            "fp64_iters_per_s": n_iter / t64, "schedule_iters_per_s": n_iter / tm, "ratio": t64 / tm}
+
+
 def trajectory_error(pkg, name, n_iter, choices):
     """The FP64, FP32 and FP32X forms from the same seeded fields for n_iter
     iterations: |dF|, max|dphiA|, max|dphiB| of each FP32-storage form against
@@ -568,6 +604,9 @@ def main():
         # the 128^3 workload keeps q(s) with sqrt(Ns) checkpointing: that mode is its headline
         log("[bench] c3 with sqrt(Ns) checkpointing (the workload's storage mode)")
         c3, _, _, _, _ = form_block(name3, ckpt=1, roof=False)
+        log("[bench] c3 200-iteration solve: FP32 then FP32X against FP64, sqrt(Ns) checkpointing")
+        c3["precision_schedule"] = precision_schedule(
+            pkg, name3, 200, 150, (c3["tiling_fp64"], c3["tiling_fp32"], c3["tiling_fp32x"]), 1)
         log("[bench] c3 with the full slice store")
         c3f, _, _, fin3, ch3 = form_block(name3, ckpt=0)
         c3["full_store"] = c3f

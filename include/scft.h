@@ -210,6 +210,15 @@ int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
    counterpart (its launch shapes are fixed, init_cuda.cu:21-80). */
 int scft_set_tiling(scft_ctx *ctx, const int *choice);
 
+/* Switch an FP32-storage context between SCFT_FP32 and SCFT_FP32X (no
+   reference counterpart).  Both forms keep the same FP32 slices, W factors and
+   FP64 fields, phi, Q, F and Anderson history, so a solve can run its bulk
+   iterations in SCFT_FP32 and finish in SCFT_FP32X, whose FP64 transform
+   arithmetic removes the FP32 form's rounding floor.  The tiling carries over
+   where the new form supports it (else the default); scft_autotune or
+   scft_set_tiling may follow.  SCFT_E_STATE for an FP64 context. */
+int scft_set_precision(scft_ctx *ctx, int precision);
+
 /* ---- heterogeneous parameter sweep (no reference counterpart: the
    reference's batch shares chiN, f, box and grid, struct.h:42-68) ----
    Independent instances with their own cubic grid n^3, box L^3, chiN and fA

@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "scft_conc_adjoint", "scft_adjoint_loop",
     "scft_sweep_create", "scft_sweep_destroy", "scft_sweep_last_error", "scft_sweep_set_fields",
     "scft_sweep_get_fields", "scft_sweep_iterate", "scft_sweep_time_iterations", "scft_sweep_autotune",
-    "scft_sweep_size", "scft_propagate_noalloc",
+    "scft_sweep_size", "scft_propagate_noalloc", "scft_set_precision",
 )
 
 
@@ -146,6 +146,7 @@ def lib():
             "scft_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_float)]),
             "scft_set_tiling": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
+            "scft_set_precision": (ctypes.c_int, [vp, ctypes.c_int]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
             # device pointers (w, q_init, q_out) and a hipStream_t
@@ -323,6 +324,10 @@ class Solver:
         """Apply a tiling choice [x, y, z variant, plane 0/1] (as autotune returns)."""
         ch = (ctypes.c_int * 4)(*[int(c) for c in choice])
         self._check(lib().scft_set_tiling(self._h, ch))
+
+    def set_precision(self, precision):
+        """Switch an FP32-storage solver between FP32 and FP32X (fields, history kept)."""
+        self._check(lib().scft_set_precision(self._h, int(precision)))
 
     def time_pass(self, axis, reps):
         ms = ctypes.c_float()

@@ -856,6 +856,10 @@ struct scft_ctx {
     double *Q = nullptr, *fe = nullptr, *ffl = nullptr;
     void *tw[3] = {nullptr, nullptr, nullptr};
     void *kf[3] = {nullptr, nullptr, nullptr};
+    // FP32-storage contexts: the twiddle and k tables of the other FP32-storage
+    // form (FP64 tables for SCFT_FP32, FP32 ones for SCFT_FP32X)
+    void *tw_alt[3] = {nullptr, nullptr, nullptr};
+    void *kf_alt[3] = {nullptr, nullptr, nullptr};
     // Anderson: rings of g = w + mix * d (wh) and of the residuals d (dh), [m+1][batch][2][N]
     double *wh = nullptr, *dh = nullptr, *G = nullptr, *coef = nullptr, *aerr = nullptr, *apart = nullptr;
     long *and_cnt = nullptr;  // device: Anderson updates done since the last set_fields
@@ -1031,28 +1035,38 @@ static int upload_tables(scft_ctx *c, const double *inst_L)
             twr[k] = (double)cosl(ang);
             twi[k] = (double)-sinl(ang);
         }
-        if (c->fp32 && !c->mixed) {
+        // FP32 arithmetic tables (SCFT_FP32) and FP64 ones (SCFT_FP64, and SCFT_FP32X, whose
+        // transforms run in FP64); an FP32-storage context holds both, the other form's in
+        // tw_alt / kf_alt (scft_set_precision swaps them)
+        void *f_tw = nullptr, *f_kf = nullptr, *d_tw = nullptr, *d_kf = nullptr;
+        if (c->fp32) {
             std::vector<float> kff(nk), tw(2 * n);
             for (size_t k = 0; k < nk; ++k) kff[k] = (float)kfa[a][k];
             for (int k = 0; k < n; ++k) {
                 tw[2 * k] = (float)twr[k];
                 tw[2 * k + 1] = (float)twi[k];
             }
-            if ((rc = dalloc(c, &c->kf[a], sizeof(float) * nk))) return rc;
-            if ((rc = dalloc(c, &c->tw[a], sizeof(float) * 2 * n))) return rc;
-            HIPX(c, hipMemcpy(c->kf[a], kff.data(), sizeof(float) * nk, hipMemcpyHostToDevice));
-            HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
-        } else {
+            if ((rc = dalloc(c, &f_kf, sizeof(float) * nk))) return rc;
+            if ((rc = dalloc(c, &f_tw, sizeof(float) * 2 * n))) return rc;
+            HIPX(c, hipMemcpy(f_kf, kff.data(), sizeof(float) * nk, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(f_tw, tw.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+        }
+        {
             std::vector<double> tw(2 * n);
             for (int k = 0; k < n; ++k) {
                 tw[2 * k] = twr[k];
                 tw[2 * k + 1] = twi[k];
             }
-            if ((rc = dalloc(c, &c->kf[a], sizeof(double) * nk))) return rc;
-            if ((rc = dalloc(c, &c->tw[a], sizeof(double) * 2 * n))) return rc;
-            HIPX(c, hipMemcpy(c->kf[a], kfa[a].data(), sizeof(double) * nk, hipMemcpyHostToDevice));
-            HIPX(c, hipMemcpy(c->tw[a], tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+            if ((rc = dalloc(c, &d_kf, sizeof(double) * nk))) return rc;
+            if ((rc = dalloc(c, &d_tw, sizeof(double) * 2 * n))) return rc;
+            HIPX(c, hipMemcpy(d_kf, kfa[a].data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+            HIPX(c, hipMemcpy(d_tw, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
         }
+        const bool ftab = c->fp32 && !c->mixed;
+        c->tw[a] = ftab ? f_tw : d_tw;
+        c->kf[a] = ftab ? f_kf : d_kf;
+        c->tw_alt[a] = ftab ? d_tw : f_tw;
+        c->kf_alt[a] = ftab ? d_kf : f_kf;
     }
     return SCFT_OK;
 }
@@ -2445,6 +2459,41 @@ extern "C" int scft_set_tiling(scft_ctx *ctx, const int *choice)
     }
     for (int a = 0; a < 3; ++a) ctx->pass_variant[a] = choice[a];
     ctx->use_plane = choice[3];
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    return SCFT_OK;
+}
+
+extern "C" int scft_set_precision(scft_ctx *ctx, int precision)
+{
+    if (!ctx) return SCFT_E_INVALID;
+    if (precision != SCFT_FP32 && precision != SCFT_FP32X)
+        return set_err(ctx, SCFT_E_INVALID, "scft_set_precision switches between SCFT_FP32 and SCFT_FP32X");
+    if (!ctx->fp32)
+        return set_err(ctx, SCFT_E_STATE, "scft_set_precision needs a context created with FP32 storage "
+                                          "(SCFT_FP32 or SCFT_FP32X)");
+    if (precision == ctx->p.precision) return SCFT_OK;
+    HIPX(ctx, hipSetDevice(ctx->device));
+    HIPX(ctx, hipStreamSynchronize(ctx->stream));
+    for (int a = 0; a < 3; ++a) {
+        std::swap(ctx->tw[a], ctx->tw_alt[a]);
+        std::swap(ctx->kf[a], ctx->kf_alt[a]);
+    }
+    ctx->mixed = precision == SCFT_FP32X;
+    ctx->p.precision = precision;
+    // the tiling carries over where the new form has it (its registers and LDS differ); else the
+    // default variant and separate x and y passes
+    if (ctx->use_plane && !plane_fits(ctx, ctx->n[0], ctx->n[1], ctx->use_plane)) ctx->use_plane = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (!ctx->active[a] || ctx->pass_variant[a] == 0) continue;
+        float t = 0.0f;
+        if (time_pass_variant(ctx, a, 1, ctx->pass_variant[a], &t) != SCFT_OK) {
+            (void)hipGetLastError();
+            ctx->pass_variant[a] = 0;
+        }
+    }
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr;

@@ -158,3 +158,34 @@ def test_fp32x_every_line_length(pkg, n):
         ephi = float(np.max(np.abs(out[FP32X][1] - out[FP64][1])))
         print(f"  {shape}: relQ={eQ:.2e} max|dphiA|={ephi:.2e}")
         assert eQ < 1e-6 and ephi < 1e-6, (shape, eQ, ephi)
+
+
+@pytest.mark.parametrize("ck", [0, 1])
+def test_set_precision_switches_forms(pkg, ck):
+    """An FP32-storage solver switched from FP32 to FP32X (and back) continues
+    exactly as a fresh solver of the new form from the same fields does (simple
+    mixing: no history), bit for bit; FP64 contexts refuse the switch."""
+    n = 16
+    p = _params(pkg, n, 4.0, 15.0, 0.45, 0.01, FP32, checkpoint=ck)
+    wA, wB = pkg.fields.seeded_fields(n, n, n, 4.0, 4.0, 4.0, "lamellar", 1.0, 0.2, 98)
+    with pkg.Solver(p) as s:
+        s.set_fields(wA, wB)
+        s.iterate(6)
+        _, _, a6, b6 = s.get_fields()
+        s.set_precision(FP32X)
+        _, fe_x = s.iterate(4)
+        _, _, a10, b10 = s.get_fields()
+        s.set_precision(FP32)
+        _, fe_back = s.iterate(3)
+    px = _params(pkg, n, 4.0, 15.0, 0.45, 0.01, FP32X, checkpoint=ck)
+    with pkg.Solver(px) as s:
+        s.set_fields(a6[0], b6[0])
+        _, fe_ref = s.iterate(4)
+    with pkg.Solver(p) as s:
+        s.set_fields(a10[0], b10[0])
+        _, fe_ref2 = s.iterate(3)
+    assert np.array_equal(fe_x, fe_ref)
+    assert np.array_equal(fe_back, fe_ref2)
+    with pkg.Solver(_params(pkg, n, 4.0, 15.0, 0.45, 0.01, FP64)) as s:
+        with pytest.raises(pkg.ScftError):
+            s.set_precision(FP32X)
