@@ -2199,7 +2199,8 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.dst = (C *)c->work;
     pa.W = (const T *)c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
-    // codes: see kPlaneCode (the boundary kernels store z(t) to a scratch slice)
+    // codes: see kPlaneCode.  A boundary kernel runs in place on the work
+    // field, as in the march, with the arguments of consecutive contour steps.
     int form = 0;
     bool fused = false;
     code_form(axis, &form, &fused);
@@ -2223,8 +2224,14 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     pa.post = 0;
     const auto *kfy = (const typename kfac<T>::type *)c->kf[1];
     const long np = (long)c->batch * c->n[2];
+    const int tmax = std::max(1, c->Ns_max - 1);
     for (int r = 0; r < reps; ++r) {
         const C *twy = (const C *)c->tw[1];
+        if (fused) {  // as in the march: contour step t's W blocks, z(t) to its own slot
+            const int t = 1 + r % tmax;
+            pa.step = t;
+            pa.slot = march_slot<T>(c, t);
+        }
         hipError_t e = form    ? run_plane<T>(c, form, pa, twy, kfy, np, fused)
                        : fused ? run_pass<T>(c, ax, c->n[ax], variant, pa, true)
                                : run_pass<T>(c, axis, c->n[axis], variant, pa, false);
