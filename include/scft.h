@@ -27,6 +27,8 @@
 #ifndef SCFT_H
 #define SCFT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -156,9 +158,11 @@ int scft_time_iterations(scft_ctx *ctx, int n_iter, float *ms);
    kernels; 8, 9, 11, 13, 15 the step-boundary forms of 3, 4, 10, 12, 14;
    16 the contour kernel over the current slice store, into scratch, which
    needs the full store: SCFT_E_INVALID in checkpoint mode)
-   on a copy of slice 0
-   between two HIP events and returns the mean duration in ms.  Leaves the
-   solver state unchanged except the scratch field. */
+   between two HIP events and returns the mean duration in ms.  A boundary
+   kernel runs as in the march: in place on the work field, launch r with the
+   arguments of contour step 1 + r % (Ns - 1), z(t) to slice t of the store.
+   Changes the work field and those slices (the next evaluation recomputes
+   both), nothing else. */
 int scft_time_pass(scft_ctx *ctx, int axis, int reps, float *ms);
 
 /* Discrete adjoint of one chain evaluation (no reference counterpart; the
@@ -264,6 +268,14 @@ int scft_sweep_size(const scft_sweep *sw, int i, long *N, int *NsA, int *NsB);
    scft_last_error(NULL). */
 int scft_propagate_noalloc(const scft_params *p, int device, int block, int sign, const double *w,
                            const double *q_init, double *q_out, void *stream);
+
+/* Attainable HBM bandwidth of `device` (no reference counterpart; the
+   measured peak beside the nominal one in a roofline): `reps` copies of
+   `bytes` bytes (a multiple of 16) by a copy kernel with 16-byte nontemporal
+   loads and stores, between HIP events, best of three timings; *gbps
+   receives (read + write bytes) / s / 1e9.  Allocates and frees two buffers
+   of `bytes`. */
+int scft_copy_bandwidth(int device, size_t bytes, int reps, double *gbps);
 
 /* Problem sizes: N points, NsA, NsB, device bytes allocated. */
 int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long *device_bytes);

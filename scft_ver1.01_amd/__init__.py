@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "scft_conc_adjoint", "scft_adjoint_loop",
     "scft_sweep_create", "scft_sweep_destroy", "scft_sweep_last_error", "scft_sweep_set_fields",
     "scft_sweep_get_fields", "scft_sweep_iterate", "scft_sweep_time_iterations", "scft_sweep_autotune",
-    "scft_sweep_size", "scft_propagate_noalloc", "scft_set_precision",
+    "scft_sweep_size", "scft_propagate_noalloc", "scft_set_precision", "scft_copy_bandwidth",
 )
 
 
@@ -147,6 +147,8 @@ def lib():
                                              ctypes.POINTER(ctypes.c_float)]),
             "scft_set_tiling": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
             "scft_set_precision": (ctypes.c_int, [vp, ctypes.c_int]),
+            "scft_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_double)]),
             "scft_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]),
             # device pointers (w, q_init, q_out) and a hipStream_t
@@ -166,6 +168,15 @@ def default_params(**kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10, device=0):
+    """Attainable HBM bandwidth (GB/s, read + write) by the library's copy kernel."""
+    g = ctypes.c_double()
+    rc = lib().scft_copy_bandwidth(int(device), int(nbytes), int(reps), ctypes.byref(g))
+    if rc:
+        raise ScftError(rc, lib().scft_last_error(None).decode())
+    return g.value
 
 
 def contour_counts(fA, ds):

@@ -1352,6 +1352,67 @@ extern "C" int scft_sizes(const scft_ctx *ctx, long *N, int *NsA, int *NsB, long
     return SCFT_OK;
 }
 
+// Copy kernel of the bandwidth probe: four 16-byte vectors per thread and
+// pass, nontemporal loads and stores (measured on MI355X: 6.1-6.2 TB/s for
+// 1 GiB, against 5.2 for the runtime's device-to-device copy).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy_probe(const f32x4 *__restrict__ a, f32x4 *__restrict__ b, long n)
+{
+    constexpr int U = 4;
+    const long stride = (long)gridDim.x * 256 * U;
+    for (long i = (long)blockIdx.x * 256 * U + threadIdx.x; i < n; i += stride) {
+        f32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i + 256L * u < n) v[u] = __builtin_nontemporal_load(a + i + 256L * u);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i + 256L * u < n) __builtin_nontemporal_store(v[u], b + i + 256L * u);
+    }
+}
+
+extern "C" int scft_copy_bandwidth(int device, size_t bytes, int reps, double *gbps)
+{
+    if (!gbps || reps < 1 || bytes < 16 || bytes % 16 != 0)
+        return set_err(nullptr, SCFT_E_INVALID, "scft_copy_bandwidth: bytes must be a positive multiple of 16, reps >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return set_err(nullptr, SCFT_E_INVALID, "scft_copy_bandwidth: no such device");
+    if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, SCFT_E_HIP, "scft_copy_bandwidth: hipSetDevice");
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const long n = (long)(bytes / 16);
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    double best = 0.0;
+    auto copy = [&]() {
+        hipLaunchKernelGGL(k_copy_probe, dim3(16384), dim3(256), 0, nullptr, (const f32x4 *)a, (f32x4 *)b, n);
+        return hipGetLastError();
+    };
+    if (e == hipSuccess) e = copy();  // untimed: code loading, first touch
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        e = hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = copy();
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.0f) best = std::max(best, 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9);
+    }
+    const hipError_t es = hipDeviceSynchronize();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("scft_copy_bandwidth: ") + hipGetErrorString(e));
+    *gbps = best;
+    return SCFT_OK;
+}
+
 extern "C" int scft_set_fields(scft_ctx *ctx, const double *wA, const double *wB)
 {
     if (!ctx) return SCFT_E_INVALID;
