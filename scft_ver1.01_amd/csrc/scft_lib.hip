@@ -675,85 +675,116 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
         for (int j = 0; j < MAXM + 2; ++j) part[(b * (MAXM + 2) + j) * nblk + blockIdx.x] = acc[j];
 }
 
-// one workgroup per instance: finish the new Gram row, solve, write coefficients
+// One workgroup per instance: finish the new Gram row, solve, write the
+// coefficients.  Every step is spread over the workgroup (a wave per Gram-row
+// sum, threads over the elements of G, U and of each elimination step); only
+// the pivot search and the back substitution run on one thread.  Each element
+// sees the floating-point operations of the serial elimination in the same
+// order, so the result is the serial one's.
 __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ part, int nblk, int maxm, int m,
                                                   const long *cnt, double *__restrict__ G, double *__restrict__ coef,
                                                   double *__restrict__ err, double ridge)
 {
-    __shared__ double sh[2 * RT / 64 + 2];
     __shared__ double row[AND_MAX + 2];
+    __shared__ double Gs[(AND_MAX + 1) * (AND_MAX + 1)];
     __shared__ double U[AND_MAX * AND_MAX];
     __shared__ double V[AND_MAX];
-    const int b = blockIdx.x;
+    __shared__ double F[AND_MAX];
+    __shared__ int piv_s, ok_s;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const AndRing rg = and_ring(cnt, m);
     const int mp = rg.mp, m1 = m + 1;
-    for (int j = 0; j < maxm + 2; ++j) {
-        if (j >= mp && j < maxm) continue;
-        const double v = finalize_sum(part + (b * (maxm + 2) + j) * nblk, nblk, sh);
-        if (threadIdx.x == 0) row[j] = v;
+    // the new Gram row (j < mp), |d|^2 (maxm) and |w|^2 (maxm + 1): one wave per sum
+    for (int q = wv; q < mp + 2; q += RT / 64) {
+        const int j = q < mp ? q : maxm + (q - mp);
+        const double *pj = part + (b * (maxm + 2) + j) * nblk;
+        double v = 0.0;
+        for (int k = lane; k < nblk; k += 64) v += pj[k];
+        v = wave_sum(v);
+        if (lane == 0) row[j] = v;
+    }
+    double *g = G + (long)b * m1 * m1;
+    for (int e = tid; e < m1 * m1; e += RT) Gs[e] = g[e];
+    __syncthreads();
+    const int c = rg.s[0];
+    if (tid < mp) {
+        const int sj = rg.s[tid + 1];
+        Gs[c * m1 + sj] = Gs[sj * m1 + c] = row[tid];
+        g[c * m1 + sj] = g[sj * m1 + c] = row[tid];
+    }
+    if (tid == 0) {
+        Gs[c * m1 + c] = g[c * m1 + c] = row[maxm];
+        err[b] = row[maxm + 1] > 0.0 ? sqrt(row[maxm] / row[maxm + 1]) : sqrt(row[maxm]);
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double *g = G + (long)b * m1 * m1;
-    const int c = rg.s[0];
-    for (int j = 0; j < mp; ++j) {
-        g[c * m1 + rg.s[j + 1]] = row[j];
-        g[rg.s[j + 1] * m1 + c] = row[j];
+    const double gcc = Gs[c * m1 + c];
+    for (int e = tid; e < mp * mp; e += RT) {
+        const int si = rg.s[e / mp + 1], sj = rg.s[e % mp + 1];
+        U[e] = gcc - Gs[c * m1 + si] - Gs[c * m1 + sj] + Gs[si * m1 + sj];
     }
-    g[c * m1 + c] = row[maxm];
-    err[b] = row[maxm + 1] > 0.0 ? sqrt(row[maxm] / row[maxm + 1]) : sqrt(row[maxm]);
-    const double gcc = g[c * m1 + c];
-    for (int i = 0; i < mp; ++i) {
-        const int si = rg.s[i + 1];
-        V[i] = gcc - g[c * m1 + si];
-        for (int j = 0; j < mp; ++j) {
-            const int sj = rg.s[j + 1];
-            U[i * mp + j] = gcc - g[c * m1 + si] - g[c * m1 + sj] + g[si * m1 + sj];
-        }
-    }
+    if (tid < mp) V[tid] = gcc - Gs[c * m1 + rg.s[tid + 1]];
+    __syncthreads();
     if (ridge > 0.0 && mp > 0) {  // U += ridge * tr(U)/mp * I
-        double tr = 0.0;
-        for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
-        const double lam = ridge * tr / mp;
-        for (int i = 0; i < mp; ++i) U[i * mp + i] += lam;
+        if (tid == 0) {
+            double tr = 0.0;
+            for (int i = 0; i < mp; ++i) tr += U[i * mp + i];
+            F[0] = ridge * tr / mp;
+        }
+        __syncthreads();
+        const double lam = F[0];
+        __syncthreads();
+        if (tid < mp) U[tid * mp + tid] += lam;
     }
     // Gaussian elimination with partial pivoting (same steps as the oracle)
-    bool ok = true;
-    for (int k = 0; k < mp && ok; ++k) {
-        int piv = k;
-        double best = fabs(U[k * mp + k]);
-        for (int i = k + 1; i < mp; ++i)
-            if (fabs(U[i * mp + k]) > best) {
-                best = fabs(U[i * mp + k]);
-                piv = i;
-            }
-        if (best == 0.0) {
-            ok = false;
-            break;
+    if (tid == 0) ok_s = 1;
+    for (int k = 0; k < mp; ++k) {
+        __syncthreads();
+        if (tid == 0) {
+            int piv = k;
+            double best = fabs(U[k * mp + k]);
+            for (int i = k + 1; i < mp; ++i)
+                if (fabs(U[i * mp + k]) > best) {
+                    best = fabs(U[i * mp + k]);
+                    piv = i;
+                }
+            if (best == 0.0) ok_s = 0;
+            piv_s = piv;
         }
+        __syncthreads();
+        if (!ok_s) break;  // uniform: read after the barrier
+        const int piv = piv_s;
         if (piv != k) {
-            for (int j = 0; j < mp; ++j) {
+            for (int j = tid; j < mp; j += RT) {
                 const double t = U[k * mp + j];
                 U[k * mp + j] = U[piv * mp + j];
                 U[piv * mp + j] = t;
             }
-            const double t = V[k];
-            V[k] = V[piv];
-            V[piv] = t;
+            if (tid == 0) {
+                const double t = V[k];
+                V[k] = V[piv];
+                V[piv] = t;
+            }
+            __syncthreads();
         }
-        for (int i = k + 1; i < mp; ++i) {
-            const double f = U[i * mp + k] / U[k * mp + k];
-            for (int j = k; j < mp; ++j) U[i * mp + j] -= f * U[k * mp + j];
-            V[i] -= f * V[k];
+        if (tid > k && tid < mp) F[tid] = U[tid * mp + k] / U[k * mp + k];
+        __syncthreads();
+        const int w = mp - k;  // columns k .. mp-1 of rows k+1 .. mp-1
+        for (int e = tid; e < (w - 1) * w; e += RT) {
+            const int i = k + 1 + e / w, j = k + e % w;
+            U[i * mp + j] -= F[i] * U[k * mp + j];
         }
+        if (tid > k && tid < mp) V[tid] -= F[tid] * V[k];
     }
-    if (ok)
+    __syncthreads();
+    const bool ok = ok_s != 0;
+    if (ok && tid == 0)
         for (int k = mp - 1; k >= 0; --k) {
             double s = V[k];
             for (int j = k + 1; j < mp; ++j) s -= U[k * mp + j] * V[j];
             V[k] = s / U[k * mp + k];
         }
-    for (int i = 0; i < AND_MAX; ++i) coef[b * AND_MAX + i] = (ok && i < mp) ? V[i] : 0.0;
+    __syncthreads();
+    for (int i = tid; i < AND_MAX; i += RT) coef[b * AND_MAX + i] = (ok && i < mp) ? V[i] : 0.0;
 }
 
 // w+ from the ring, then F partials with the updated fields
