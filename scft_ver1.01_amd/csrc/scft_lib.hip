@@ -1510,7 +1510,9 @@ static long ck_remarch_steps(int Ns, int K)
 //          the march-and-contour's window, segment buffers, carry and
 //          ping-pong pairs (enqueue_march_contour_ck), then by the adjoint's
 //          pair walk as D[i] = nck + i, A[i] = nck + K + 2 + i (i = 1..K-1)
-//          and ZB[j] = nck + 2K + 2 + j
+//          and ZB[j] = nck + 2K + 2 + j; then the march's opened states
+//          O(jK) at nck + 2K + 4 + j (j < J = (Ns/2)/K) and the re-marches'
+//          work field at nck + 2K + 4 + J
 // The spacing K is the one, within [K0/2, 2 K0] for K0 = ceil(sqrt(Ns+1)),
 // of the lowest ck_remarch_steps with a store no larger than K0's (Ns = 200:
 // K = 15, 84 re-marched steps; Ns = 100: K = 11, 40).
@@ -1532,7 +1534,8 @@ static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
     const int nck = Ns / K + 1;
     if (K_out) *K_out = K;
     if (nck_out) *nck_out = nck;
-    return ckpt ? (long)nck + 2 * K + 4 : (long)Ns + 3;
+    // + the march's opened states O(jK), j < J = (Ns/2)/K, and the re-marches' work field
+    return ckpt ? (long)nck + 2 * K + 4 + (Ns / 2) / K + 1 : (long)Ns + 3;
 }
 
 // block of the real (br) and imaginary (bi) part at packed step t (1..Ns):
@@ -1652,6 +1655,54 @@ static typename cplx<T>::type *march_slot(scft_ctx *c, int t)
     return slot_ptr<T>(c, c->ck_n + c->ck_K + (t & 1));
 }
 
+// The opened state O(t) of the step-boundary march is the field a boundary
+// launch leaves after it closes step t: z(t) times step t+1's pre factor with
+// step t+1's first unit applied (O(0): the first launch's output).  A chunk
+// can start from O(t0) instead of z(t0) (src_open) and skip the plain launch
+// that would recompute it, and can end with the boundary launch that closes
+// its last step (end_open, its O(t_end) left in wk) instead of a plain one; a
+// march started from a saved O(t0) then reproduces the uninterrupted march
+// bit for bit.  open_of(t) (t0 <= t, may return nullptr) names a buffer where
+// O(t) is kept (the next launch reads it there); wk is the in-place work
+// buffer of the chunk.
+template <typename T, typename DstOf, typename OpenOf>
+static hipError_t enqueue_march_fused_ex(scft_ctx *c, const typename cplx<T>::type *src0, bool src_open, int t0,
+                                         int nsteps, DstOf dst_of, OpenOf open_of, typename cplx<T>::type *wk,
+                                         bool end_open, typename cplx<T>::type **last)
+{
+    using C = typename cplx<T>::type;
+    int units[3];
+    const int nu = step_units(c, units);
+    C *dst = nullptr;
+    const C *cur = src0;  // the field the next launch reads
+    const int t_end = t0 + nsteps;
+    for (int t = t0 + 1; t <= t_end; ++t) {
+        dst = dst_of(t);
+        for (int k = 0; k < nu; ++k) {
+            const int u = (t & 1) ? units[k] : units[nu - 1 - k];
+            const bool first = k == 0, lastu = k == nu - 1;
+            if (first && (t > t0 + 1 || src_open)) continue;  // opened by the launch that closed t-1
+            hipError_t e;
+            if (lastu && (t < t_end || end_open)) {
+                C *o = open_of(t);
+                C *out = o ? o : wk;
+                PassArgs<T> pa = pass_args<T>(c, u, cur, out, t, 0, 0, first, false);
+                pa.slot = dst;
+                e = launch_unit<T>(c, u, pa, true);
+                cur = out;
+            } else {
+                C *o = (first && t == t0 + 1) ? open_of(t0) : nullptr;  // O(t0) of a chunk opened from z(t0)
+                C *out = lastu ? dst : (o ? o : wk);
+                e = launch_unit<T>(c, u, pass_args<T>(c, u, cur, out, t, 0, 0, first, lastu), false);
+                cur = out;
+            }
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (last) *last = dst;
+    return hipSuccess;
+}
+
 // The packed march with step-boundary kernels.  S = W C_x C_y C_z W and the
 // axis convolutions commute, so odd steps run the units in forward order and
 // even steps in reverse: the last unit of step t is the first of step t+1, and
@@ -1660,40 +1711,19 @@ static typename cplx<T>::type *march_slot(scft_ctx *c, int t)
 // z(t0) = src marched to z(t0 + nsteps); z(t) goes to dst_of(t).  The order
 // follows t itself, not t - t0, so a march split into chunks, or re-marched
 // from a checkpoint, runs its steps' units in the order the march from z(0)
-// does.  Only a chunk's first step can round differently: there a plain
-// kernel runs the first unit where the march from z(0) has a boundary kernel
-// (an xy-plane boundary kernel opens its step y then x, a plain plane kernel
-// runs x then y; FP32X rounds the carried slice to FP32).
+// does.  A chunk started from a slice has one launch the march from z(0) does
+// not: a plain kernel runs the first unit where that march has a boundary
+// kernel (an xy-plane boundary kernel opens its step y then x, a plain plane
+// kernel runs x then y; FP32X rounds the carried slice to FP32), so its
+// slices can differ in the last bits; enqueue_march_fused_ex started from a
+// saved opened state has no such launch.
 template <typename T, typename DstOf>
 static hipError_t enqueue_march_fused_from(scft_ctx *c, const typename cplx<T>::type *src0, int t0, int nsteps,
                                            DstOf dst_of, typename cplx<T>::type **last)
 {
     using C = typename cplx<T>::type;
-    C *work = (C *)c->work;
-    int units[3];
-    const int nu = step_units(c, units);
-    C *dst = nullptr;
-    const int t_end = t0 + nsteps;
-    for (int t = t0 + 1; t <= t_end; ++t) {
-        dst = dst_of(t);
-        for (int k = 0; k < nu; ++k) {
-            const int u = (t & 1) ? units[k] : units[nu - 1 - k];
-            const bool first = k == 0, lastu = k == nu - 1;
-            if (first && t > t0 + 1) continue;  // opened by step t-1's boundary kernel
-            const C *src = (first && t == t0 + 1) ? src0 : work;
-            hipError_t e;
-            if (lastu && t < t_end) {
-                PassArgs<T> pa = pass_args<T>(c, u, src, work, t, 0, 0, first, false);
-                pa.slot = dst;
-                e = launch_unit<T>(c, u, pa, true);
-            } else {
-                e = launch_unit<T>(c, u, pass_args<T>(c, u, src, lastu ? dst : work, t, 0, 0, first, lastu), false);
-            }
-            if (e != hipSuccess) return e;
-        }
-    }
-    if (last) *last = dst;
-    return hipSuccess;
+    return enqueue_march_fused_ex<T>(c, src0, false, t0, nsteps, dst_of, [](int) -> C * { return nullptr; },
+                                     (C *)c->work, false, last);
 }
 
 template <typename T>
@@ -1805,18 +1835,29 @@ struct PairWalk {
 // That is J (K - 1), about Ns/2, re-marched steps per evaluation.  Q comes
 // from z(Ns) as soon as the march reaches it, before the last segment's
 // pairs, whose launch applies the ds/Q factor to the summed phi.
+// With step-boundary kernels the march keeps its opened state in the work
+// field between its chunks (the re-marches have a work field of their own)
+// and keeps O(jK), j < J, beside the checkpoints: each re-march starts from
+// O(jK) and ends open, so no chunk has a plain first or last launch and every
+// re-marched slice equals the march's own bit for bit.
 // Slots: checkpoint k at k; window z(JK + 1 + i) and the re-marched z(jK + i)
 // at nck + 1 + i and nck + i; the march's z(Ns - jK - K + 1 + i) at nck + K + i;
 // each chunk's last slice in the carry pair nck + 2K, nck + 2K + 1; the steps
-// before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3.
+// before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3; O(jK)
+// at nck + 2K + 4 + j and the re-marches' work field at nck + 2K + 4 + J.
 template <typename T>
 static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *phiB)
 {
     using C = typename cplx<T>::type;
     const int Ns = c->Ns, K = c->ck_K, nck = c->ck_n, half = Ns / 2, J = half / K;
-    const long carry0 = (long)nck + 2L * K, pp0 = carry0 + 2;
+    const long carry0 = (long)nck + 2L * K, pp0 = carry0 + 2, open0 = carry0 + 4;
     const dim3 gc((unsigned)((c->N + RT - 1) / RT), c->batch);
     auto ck = [&](int t) { return slot_ptr<T>(c, t / K); };
+    int units[3];
+    const bool fused = c->p.fuse_steps && step_units(c, units) >= 2;
+    C *const wk1 = (C *)c->work, *const wk2 = slot_ptr<T>(c, open0 + J);
+    auto open_ck = [&](int t) -> C * { return (t % K == 0 && t / K < J) ? slot_ptr<T>(c, open0 + t / K) : nullptr; };
+    auto none = [](int) -> C * { return nullptr; };
     int launched = 0;
     const int P = contour_points<T>(c->N);  // points per thread (16-byte loads of each plane)
     const dim3 gcp((unsigned)((c->N / P + RT - 1) / RT), c->batch);
@@ -1861,7 +1902,9 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
         if (t >= wlo) return slot_ptr<T>(c, nck + 1 + (t - wlo));
         return slot_ptr<T>(c, pp0 + (t & 1));
     };
-    hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, 0), 0, whi, first_dst);
+    hipError_t e = fused ? enqueue_march_fused_ex<T>(c, slot_ptr<T>(c, 0), false, 0, whi, first_dst, open_ck, wk1,
+                                                     J > 0, nullptr)
+                         : enqueue_march_from<T>(c, slot_ptr<T>(c, 0), 0, whi, first_dst);
     if (e != hipSuccess) return e;
     auto first_at = [&](int t) -> const C * { return first_dst(t); };
     if (J == 0 && (e = q_of(first_at(Ns))) != hipSuccess) return e;  // the first chunk reached z(Ns)
@@ -1871,18 +1914,25 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     for (int j = J - 1; j >= 0; --j) {
         const int tau0 = j * K, tau1 = tau0 + K - 1, dlo = Ns - tau1, dhi = Ns - tau0;
         const long next = carry == carry0 ? carry0 + 1 : carry0;
-        const C *src = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
         // z(dlo .. dhi): the march's next K steps (tcur = dlo - 1)
         auto ddst = [&](int t) -> C * {
             if (t % K == 0) return ck(t);
             if (t == dhi) return slot_ptr<T>(c, next);
             return slot_ptr<T>(c, nck + K + (t - dlo));
         };
-        if ((e = enqueue_march_from<T>(c, src, tcur, dhi - tcur, ddst)) != hipSuccess) return e;
+        if (fused) {  // on from the march's opened state O(tcur) in wk1
+            e = enqueue_march_fused_ex<T>(c, wk1, true, tcur, dhi - tcur, ddst, none, wk1, j > 0, nullptr);
+        } else {
+            const C *src = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
+            e = enqueue_march_from<T>(c, src, tcur, dhi - tcur, ddst);
+        }
+        if (e != hipSuccess) return e;
         if (j == 0 && (e = q_of(ddst(Ns))) != hipSuccess) return e;  // dhi == Ns
         // z(tau0 + 1 .. tau1) re-marched from checkpoint j
         auto adst = [&](int t) -> C * { return slot_ptr<T>(c, nck + (t - tau0)); };
-        if ((e = enqueue_march_from<T>(c, ck(tau0), tau0, K - 1, adst)) != hipSuccess) return e;
+        e = fused ? enqueue_march_fused_ex<T>(c, open_ck(tau0), true, tau0, K - 1, adst, none, wk2, true, nullptr)
+                  : enqueue_march_from<T>(c, ck(tau0), tau0, K - 1, adst);
+        if (e != hipSuccess) return e;
         auto at = [&](int t) -> const C * { return t == tau0 ? ck(t) : adst(t); };
         auto dt = [&](int t) -> const C * { return ddst(t); };
         if ((e = pairs(tau0, tau1, at, dt, j == 0)) != hipSuccess) return e;
