@@ -659,6 +659,9 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
         wc[N + r] = c + mix * dB;
         dc[r] = dA;
         dc[N + r] = dB;
+        // one round trip per history slot: issuing all of them at once
+        // (unconditional loads, slots past the history re-read) measured
+        // slower at 128^3 with m = 20 (256 against 212 us)
 #pragma unroll
         for (int j = 0; j < MAXM; ++j) {
             if (j < rg.mp) {
@@ -676,11 +679,11 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
 }
 
 // One workgroup per instance: finish the new Gram row, solve, write the
-// coefficients.  Every step is spread over the workgroup (a wave per Gram-row
-// sum, threads over the elements of G, U and of each elimination step); only
-// the pivot search and the back substitution run on one thread.  Each element
-// sees the floating-point operations of the serial elimination in the same
-// order, so the result is the serial one's.
+// coefficients.  The Gram-row sums take a wave each and the elements of G
+// and U are spread over the workgroup; the elimination runs in the first wave
+// (a lane per row, the pivot found by a wave argmax) and the back
+// substitution on one lane.  Each element sees the floating-point operations
+// of the serial elimination in the same order.
 __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ part, int nblk, int maxm, int m,
                                                   const long *cnt, double *__restrict__ G, double *__restrict__ coef,
                                                   double *__restrict__ err, double ridge)
@@ -690,7 +693,7 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
     __shared__ double U[AND_MAX * AND_MAX];
     __shared__ double V[AND_MAX];
     __shared__ double F[AND_MAX];
-    __shared__ int piv_s, ok_s;
+    __shared__ int ok_s;
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const AndRing rg = and_ring(cnt, m);
     const int mp = rg.mp, m1 = m + 1;
@@ -699,7 +702,8 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
         const int j = q < mp ? q : maxm + (q - mp);
         const double *pj = part + (b * (maxm + 2) + j) * nblk;
         double v = 0.0;
-        for (int k = lane; k < nblk; k += 64) v += pj[k];
+#pragma unroll 8
+        for (int k = lane; k < nblk; k += 64) v += pj[k];  // eight loads in flight, the sum in order
         v = wave_sum(v);
         if (lane == 0) row[j] = v;
     }
@@ -735,55 +739,65 @@ __global__ void __launch_bounds__(RT) k_and_solve(const double *__restrict__ par
         __syncthreads();
         if (tid < mp) U[tid * mp + tid] += lam;
     }
-    // Gaussian elimination with partial pivoting (same steps as the oracle)
-    if (tid == 0) ok_s = 1;
-    for (int k = 0; k < mp; ++k) {
-        __syncthreads();
-        if (tid == 0) {
-            int piv = k;
-            double best = fabs(U[k * mp + k]);
-            for (int i = k + 1; i < mp; ++i)
-                if (fabs(U[i * mp + k]) > best) {
-                    best = fabs(U[i * mp + k]);
-                    piv = i;
+    // Gaussian elimination with partial pivoting (the oracle's steps) in the
+    // first wave alone: lane i holds row i, the pivot is a wave argmax and the
+    // LDS updates need only wave-scope ordering (with the whole workgroup,
+    // every step took four barriers and a serial pivot search)
+    __syncthreads();  // U and V are complete
+    if (wv == 0) {
+        auto wave_sync = [] {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        int ok = 1;
+        for (int k = 0; k < mp; ++k) {
+            // the largest |U[i][k]|, i >= k; the first such row on ties
+            double best = (lane >= k && lane < mp) ? fabs(U[lane * mp + k]) : -1.0;
+            int piv = lane;
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int op = __shfl_xor(piv, off);
+                if (ob > best || (ob == best && op < piv)) {
+                    best = ob;
+                    piv = op;
                 }
-            if (best == 0.0) ok_s = 0;
-            piv_s = piv;
-        }
-        __syncthreads();
-        if (!ok_s) break;  // uniform: read after the barrier
-        const int piv = piv_s;
-        if (piv != k) {
-            for (int j = tid; j < mp; j += RT) {
-                const double t = U[k * mp + j];
-                U[k * mp + j] = U[piv * mp + j];
-                U[piv * mp + j] = t;
             }
-            if (tid == 0) {
-                const double t = V[k];
-                V[k] = V[piv];
-                V[piv] = t;
+            if (best == 0.0) {  // uniform: every lane holds the reduced value
+                ok = 0;
+                break;
             }
-            __syncthreads();
+            if (piv != k) {
+                if (lane < mp) {
+                    const double t = U[k * mp + lane];
+                    U[k * mp + lane] = U[piv * mp + lane];
+                    U[piv * mp + lane] = t;
+                }
+                if (lane == 0) {
+                    const double t = V[k];
+                    V[k] = V[piv];
+                    V[piv] = t;
+                }
+                wave_sync();
+            }
+            if (lane > k && lane < mp) {
+                const double f = U[lane * mp + k] / U[k * mp + k];
+#pragma unroll 4
+                for (int j = k; j < mp; ++j) U[lane * mp + j] -= f * U[k * mp + j];
+                V[lane] -= f * V[k];
+            }
+            wave_sync();
         }
-        if (tid > k && tid < mp) F[tid] = U[tid * mp + k] / U[k * mp + k];
-        __syncthreads();
-        const int w = mp - k;  // columns k .. mp-1 of rows k+1 .. mp-1
-        for (int e = tid; e < (w - 1) * w; e += RT) {
-            const int i = k + 1 + e / w, j = k + e % w;
-            U[i * mp + j] -= F[i] * U[k * mp + j];
-        }
-        if (tid > k && tid < mp) V[tid] -= F[tid] * V[k];
+        if (ok && lane == 0)
+            for (int k = mp - 1; k >= 0; --k) {
+                double sum = V[k];
+                for (int j = k + 1; j < mp; ++j) sum -= U[k * mp + j] * V[j];
+                V[k] = sum / U[k * mp + k];
+            }
+        if (lane == 0) ok_s = ok;
     }
     __syncthreads();
     const bool ok = ok_s != 0;
-    if (ok && tid == 0)
-        for (int k = mp - 1; k >= 0; --k) {
-            double s = V[k];
-            for (int j = k + 1; j < mp; ++j) s -= U[k * mp + j] * V[j];
-            V[k] = s / U[k * mp + k];
-        }
-    __syncthreads();
     for (int i = tid; i < AND_MAX; i += RT) coef[b * AND_MAX + i] = (ok && i < mp) ? V[i] : 0.0;
 }
 
@@ -808,6 +822,8 @@ __global__ void __launch_bounds__(RT) k_and_update(double *__restrict__ wA, doub
     for (long r = (long)blockIdx.x * RT + threadIdx.x; r < N; r += (long)nblk * RT) {
         const double ga0 = g0[r], gb0 = g0[N + r];
         double a = ga0, cB = gb0;
+        // four history slots' loads in flight per round (the sums keep their order)
+#pragma unroll 4
         for (int k = 0; k < mp; ++k) {
             const double *gk = gh + rg.s[k + 1] * slot + b * 2 * N;
             const double c = cs[k];
@@ -2006,18 +2022,16 @@ static hipError_t enqueue_update(scft_ctx *c)
                            (const double *)c->chiN_b, c->p.wopt, c->p.wcmp, c->fpart);
     } else {
         const int m = c->p.anderson_m;
-        if (c->and_maxm == 8)
-            hipLaunchKernelGGL(k_and_resid<8>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
-                               c->apart);
-        else if (c->and_maxm == 16)
-            hipLaunchKernelGGL(k_and_resid<16>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
-                               c->apart);
-        else
-            hipLaunchKernelGGL(k_and_resid<32>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N,
-                               c->batch, c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,
-                               c->apart);
+#define SCFT_AND_RESID(MM)                                                                                     \
+    hipLaunchKernelGGL(k_and_resid<MM>, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch, \
+                       c->nblk, (const double *)c->chiN_b, c->p.wcmp, c->p.anderson_mix, c->wh, c->dh, c->and_cnt, m,  \
+                       c->apart)
+        switch (c->and_maxm) {
+        case 8: SCFT_AND_RESID(8); break;
+        case 16: SCFT_AND_RESID(16); break;
+        default: SCFT_AND_RESID(32); break;
+        }
+#undef SCFT_AND_RESID
         hipLaunchKernelGGL(k_and_solve, dim3(c->batch), dim3(RT), 0, c->stream, c->apart, c->nblk, c->and_maxm, m,
                            c->and_cnt, c->G, c->coef, c->aerr, c->p.anderson_reg);
         hipLaunchKernelGGL(k_and_update, g, dim3(RT), 0, c->stream, c->wA, c->wB, c->phiA, c->phiB, c->N, c->batch,
