@@ -267,10 +267,27 @@ template <typename T, int P> struct alignas(P * sizeof(T)) RGroup {
 template <typename T, int P> struct ZGroup {
     RGroup<T, P> re, im;
 };
+// The contour reads every slice once: nontemporal 16-byte loads, so that the
+// slices streaming through do not evict what the next iteration's march reads
+// again (measured 64^3 FP32: contour 75 -> 64 us, iteration 1.69 -> 1.61 ms;
+// 128^3 FP64 contour 1253 -> 1154 us)
+template <typename T, int P>
+__device__ __forceinline__ RGroup<T, P> ld_stream(const RGroup<T, P> *p)
+{
+    if constexpr (sizeof(RGroup<T, P>) == 16) {
+        using V = __attribute__((ext_vector_type(4))) unsigned int;
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+        RGroup<T, P> r;
+        __builtin_memcpy(&r, &v, 16);
+        return r;
+    } else {
+        return *p;
+    }
+}
 template <typename T, int P>
 __device__ __forceinline__ ZGroup<T, P> ld_group(const RGroup<T, P> *re, long im_off, long gi)
 {
-    return ZGroup<T, P>{re[gi], re[im_off + gi]};
+    return ZGroup<T, P>{ld_stream<T, P>(re + gi), ld_stream<T, P>(re + im_off + gi)};
 }
 
 // phi by the trapezoid rule with FP64 accumulation (w_to_phi_go, cuda_aid.cu:61-118).
