@@ -676,14 +676,25 @@ __global__ void __launch_bounds__(RT) k_and_resid(const double *__restrict__ wA,
         wc[N + r] = c + mix * dB;
         dc[r] = dA;
         dc[N + r] = dB;
-        // one round trip per history slot: issuing all of them at once
-        // (unconditional loads, slots past the history re-read) measured
-        // slower at 128^3 with m = 20 (256 against 212 us)
+        // the history slots in groups of four, each group's loads issued
+        // together; a group's slots past the history re-read its first slot
+        // (a cache hit) and add nothing.  Measured at 128^3 with m = 20: one
+        // round trip per slot 217 us, groups of four 178 us, all 32 slots'
+        // loads at once 256 us (the slots past the history read from memory)
 #pragma unroll
-        for (int j = 0; j < MAXM; ++j) {
-            if (j < rg.mp) {
-                const double *dj = dh + rg.s[j + 1] * slot + b * 2 * N;
-                acc[j] += dA * dj[r] + dB * dj[N + r];
+        for (int j0 = 0; j0 < MAXM; j0 += 4) {
+            if (j0 < rg.mp) {
+                double xa[4], xb[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sj = (j0 + u < rg.mp) ? rg.s[j0 + u + 1] : rg.s[j0 + 1];
+                    const double *dj = dh + sj * slot + b * 2 * N;
+                    xa[u] = dj[r];
+                    xb[u] = dj[N + r];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j0 + u < rg.mp) acc[j0 + u] += dA * xa[u] + dB * xb[u];
             }
         }
         acc[MAXM] += dA * dA + dB * dB;
@@ -1293,7 +1304,6 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
             c->last_axis = a;
         }
     }
-    // reduction blocks per instance: ~2 points per thread, at most 2048 partials
     {
         if (p.xy_plane < -1 || p.xy_plane > PLANE_FORMS)
             return set_err(nullptr, SCFT_E_INVALID, "xy_plane must be -1.." + std::to_string(PLANE_FORMS));
@@ -1309,7 +1319,11 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
         c->pass_variant[a] = p.tiling < 0 ? 0 : p.tiling;
         if (c->pass_variant[a] == 2 && !pass3_supported(n[a])) c->pass_variant[a] = 0;
     }
-    c->nblk = (int)std::min<long>(2048, std::max<long>(1, (c->N + RT * 2 - 1) / (RT * 2)));
+    // blocks of the point-wise and reduction kernels: two points per thread,
+    // at most 1024 blocks (eight points per thread at 128^3: fewer block
+    // partials for the Anderson Gram row and F, 2048 blocks measured 1.4 %
+    // slower per 128^3 iteration)
+    c->nblk = (int)std::min<long>(1024, std::max<long>(1, (c->N + RT * 2 - 1) / (RT * 2)));
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
