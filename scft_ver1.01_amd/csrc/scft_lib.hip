@@ -1558,8 +1558,8 @@ static long ck_remarch_steps(int Ns, int K)
 //          ping-pong pairs (enqueue_march_contour_ck), then by the adjoint's
 //          pair walk as D[i] = nck + i, A[i] = nck + K + 2 + i (i = 1..K-1)
 //          and ZB[j] = nck + 2K + 2 + j; then the march's opened states
-//          O(jK) at nck + 2K + 4 + j (j < J = (Ns/2)/K) and the re-marches'
-//          work field at nck + 2K + 4 + J
+//          O(kK) at nck + 2K + 4 + k (k < nck, kK < Ns) and the re-marches'
+//          work field at nck + 2K + 4 + nck
 // The spacing K is the one, within [K0/2, 2 K0] for K0 = ceil(sqrt(Ns+1)),
 // of the lowest ck_remarch_steps with a store no larger than K0's (Ns = 200:
 // K = 15, 84 re-marched steps; Ns = 100: K = 11, 40).
@@ -1581,8 +1581,8 @@ static long slot_count(int Ns, int ckpt, int *K_out, int *nck_out)
     const int nck = Ns / K + 1;
     if (K_out) *K_out = K;
     if (nck_out) *nck_out = nck;
-    // + the march's opened states O(jK), j < J = (Ns/2)/K, and the re-marches' work field
-    return ckpt ? (long)nck + 2 * K + 4 + (Ns / 2) / K + 1 : (long)Ns + 3;
+    // + the march's opened states O(kK), k < nck, and the re-marches' work field
+    return ckpt ? (long)nck + 2 * K + 4 + nck + 1 : (long)Ns + 3;
 }
 
 // block of the real (br) and imaginary (bi) part at packed step t (1..Ns):
@@ -1828,13 +1828,31 @@ static hipError_t enqueue_march_from(scft_ctx *c, const typename cplx<T>::type *
 // re-marched to z(jK + 1 .. jK + K - 1), and when Ns - tau enters segment k
 // (s = kK .. kK + K - 1) the descending one is re-marched from checkpoint k --
 // about 2 Ns extra steps for a full traversal, each segment one step-boundary
-// march, with O(sqrt Ns) slices held.
+// march, with O(sqrt Ns) slices held.  With step-boundary kernels a segment
+// is re-marched from the forward march's opened state O(kK), kept beside
+// checkpoint k, and ends open in the work field (a plain last launch where it
+// reaches Ns, as the forward march's does), so its slices equal the forward
+// march's bit for bit (from z(kK) a plain first launch would round
+// differently).
 template <typename T>
 struct PairWalk {
     using C = typename cplx<T>::type;
     scft_ctx *c;
     int seg = -1;
     explicit PairWalk(scft_ctx *cc) : c(cc) {}
+
+    template <typename DstOf>
+    hipError_t remarch(int k, int len, DstOf dst_of)
+    {
+        const int K = c->ck_K, nck = c->ck_n;
+        int units[3];
+        if (len <= 0) return hipSuccess;
+        if (c->p.fuse_steps && step_units(c, units) >= 2)
+            return enqueue_march_fused_ex<T>(c, slot_ptr<T>(c, (long)nck + 2L * K + 4 + k), true, k * K, len, dst_of,
+                                             [](int) -> C * { return nullptr; }, (C *)c->work, k * K + len < c->Ns,
+                                             nullptr);
+        return enqueue_march_from<T>(c, slot_ptr<T>(c, k), k * K, len, dst_of);
+    }
 
     hipError_t next(int tau, const C **zt, const C **zu)
     {
@@ -1849,8 +1867,7 @@ struct PairWalk {
         const int j = tau / K, ia = tau - j * K;
         if (ia == 0) {
             const int len = std::min(K - 1, Ns - j * K);
-            hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, j), j * K, len,
-                                                 [&](int t) { return slot_ptr<T>(c, a0 + (t - j * K)); });
+            hipError_t e = remarch(j, len, [&](int t) { return slot_ptr<T>(c, a0 + (t - j * K)); });
             if (e != hipSuccess) return e;
         }
         *zt = (ia == 0) ? slot_ptr<T>(c, j) : slot_ptr<T>(c, a0 + ia);
@@ -1858,8 +1875,7 @@ struct PairWalk {
         const int s = Ns - tau, k = s / K;
         if (k != seg) {
             const int len = std::min(K - 1, Ns - k * K);
-            hipError_t e = enqueue_march_from<T>(c, slot_ptr<T>(c, k), k * K, len,
-                                                 [&](int t) { return slot_ptr<T>(c, nck + (t - k * K)); });
+            hipError_t e = remarch(k, len, [&](int t) { return slot_ptr<T>(c, nck + (t - k * K)); });
             if (e != hipSuccess) return e;
             seg = k;
         }
@@ -1884,14 +1900,15 @@ struct PairWalk {
 // pairs, whose launch applies the ds/Q factor to the summed phi.
 // With step-boundary kernels the march keeps its opened state in the work
 // field between its chunks (the re-marches have a work field of their own)
-// and keeps O(jK), j < J, beside the checkpoints: each re-march starts from
-// O(jK) and ends open, so no chunk has a plain first or last launch and every
-// re-marched slice equals the march's own bit for bit.
+// and keeps O(kK) beside every checkpoint: each re-march (here and in the
+// adjoint's pair walk) starts from O(kK) and ends open, so no chunk has a
+// plain first or last launch and every re-marched slice equals the march's
+// own bit for bit.
 // Slots: checkpoint k at k; window z(JK + 1 + i) and the re-marched z(jK + i)
 // at nck + 1 + i and nck + i; the march's z(Ns - jK - K + 1 + i) at nck + K + i;
 // each chunk's last slice in the carry pair nck + 2K, nck + 2K + 1; the steps
 // before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3; O(jK)
-// at nck + 2K + 4 + j and the re-marches' work field at nck + 2K + 4 + J.
+// at nck + 2K + 4 + k and the re-marches' work field at nck + 2K + 4 + nck.
 template <typename T>
 static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *phiB)
 {
@@ -1902,8 +1919,10 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     auto ck = [&](int t) { return slot_ptr<T>(c, t / K); };
     int units[3];
     const bool fused = c->p.fuse_steps && step_units(c, units) >= 2;
-    C *const wk1 = (C *)c->work, *const wk2 = slot_ptr<T>(c, open0 + J);
-    auto open_ck = [&](int t) -> C * { return (t % K == 0 && t / K < J) ? slot_ptr<T>(c, open0 + t / K) : nullptr; };
+    C *const wk1 = (C *)c->work, *const wk2 = slot_ptr<T>(c, open0 + nck);
+    auto open_ck = [&](int t) -> C * { return (t % K == 0 && t < Ns) ? slot_ptr<T>(c, open0 + t / K) : nullptr; };
+    // where a chunk that ends open at t leaves O(t): its checkpoint's slot, else the work field
+    auto open_at = [&](int t) -> C * { return open_ck(t) ? open_ck(t) : wk1; };
     auto none = [](int) -> C * { return nullptr; };
     int launched = 0;
     const int P = contour_points<T>(c->N);  // points per thread (16-byte loads of each plane)
@@ -1957,6 +1976,7 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
     if (J == 0 && (e = q_of(first_at(Ns))) != hipSuccess) return e;  // the first chunk reached z(Ns)
     if ((e = pairs(wlo, half, first_at, first_at, J == 0)) != hipSuccess) return e;
     int tcur = whi;
+    C *ocur = open_at(whi);  // O(tcur), with step-boundary kernels
     long carry = carry0;
     for (int j = J - 1; j >= 0; --j) {
         const int tau0 = j * K, tau1 = tau0 + K - 1, dlo = Ns - tau1, dhi = Ns - tau0;
@@ -1967,8 +1987,9 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
             if (t == dhi) return slot_ptr<T>(c, next);
             return slot_ptr<T>(c, nck + K + (t - dlo));
         };
-        if (fused) {  // on from the march's opened state O(tcur) in wk1
-            e = enqueue_march_fused_ex<T>(c, wk1, true, tcur, dhi - tcur, ddst, none, wk1, j > 0, nullptr);
+        if (fused) {  // on from the march's opened state O(tcur)
+            e = enqueue_march_fused_ex<T>(c, ocur, true, tcur, dhi - tcur, ddst, open_ck, wk1, j > 0, nullptr);
+            ocur = open_at(dhi);
         } else {
             const C *src = (tcur % K == 0) ? ck(tcur) : slot_ptr<T>(c, carry);
             e = enqueue_march_from<T>(c, src, tcur, dhi - tcur, ddst);
@@ -3194,10 +3215,15 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
     const size_t n_traj = (size_t)K * S;
     const size_t n_part = (size_t)nbl * nb * LMAXE, n_red = (size_t)nb * LMAXE;
     const size_t n_coef = (size_t)K * nb * AND_MAX, n_rv = (size_t)K * nb * (AND_MAX + 1), n_q = (size_t)K * nb;
-    // checkpoint mode: each iteration's checkpoints z(jK) (the slice store's first ck_n slots), so that the
+    // checkpoint mode: each iteration's checkpoints z(kK) (the slice store's first ck_n slots) and, with
+    // step-boundary kernels, the march's opened states O(kK) (ck_n slots from nck + 2K + 4), so that the
     // reverse sweep restores the chain's forward state instead of recomputing it (march and contour)
     const bool keep_ck = grad && ctx->ckpt;
-    const size_t ck_bytes = (size_t)ctx->ck_n * BN * ctx->csize, n_ck = keep_ck ? (size_t)K * ck_bytes / 8 : 0;
+    int units3[3];
+    const bool keep_open = keep_ck && ctx->p.fuse_steps && step_units(ctx, units3) >= 2;
+    const size_t ck_bytes = (size_t)ctx->ck_n * BN * ctx->csize, ck_rec = (keep_open ? 2 : 1) * ck_bytes;
+    const size_t n_ck = keep_ck ? (size_t)K * ck_rec / 8 : 0;
+    char *const open_slots = (char *)ctx->slots + ((size_t)ctx->ck_n + 2 * ctx->ck_K + 4) * BN * ctx->csize;
     const size_t total =
         (grad ? 5 : 3) * n_traj + 2 * S + 3 * (size_t)BN + n_part + n_red + n_coef + n_rv + n_q + n_ck;
     if ((rc = ensure_loop_buffer(ctx, total * sizeof(double)))) return rc;
@@ -3272,7 +3298,9 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
         HIPX(ctx, enqueue_conc_scratch(ctx));
         HIPX(ctx, hipMemcpyAsync(phk, dev + 6 * BN, 8 * S, hipMemcpyDeviceToDevice, s));
         HIPX(ctx, hipMemcpyAsync(Qk + (size_t)k * nb, ctx->Q, 8 * nb, hipMemcpyDeviceToDevice, s));
-        if (keep_ck) HIPX(ctx, hipMemcpyAsync(CK + (size_t)k * ck_bytes, ctx->slots, ck_bytes, hipMemcpyDeviceToDevice, s));
+        if (keep_ck) HIPX(ctx, hipMemcpyAsync(CK + (size_t)k * ck_rec, ctx->slots, ck_bytes, hipMemcpyDeviceToDevice, s));
+        if (keep_open)
+            HIPX(ctx, hipMemcpyAsync(CK + (size_t)k * ck_rec + ck_bytes, open_slots, ck_bytes, hipMemcpyDeviceToDevice, s));
         if (k == K - 1) break;
         double *ddk = DD + (size_t)k * S;
         hipLaunchKernelGGL(k_loop_resid, dim3(g1), dim3(RT), 0, s, (const double *)wtk, (const double *)phk, ddk, chiN,
@@ -3388,9 +3416,12 @@ extern "C" int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w
         // chain adjoint: lam = pbar at u_k = (wt_k,A + h, wt_k,B)
         hipLaunchKernelGGL(k_loop_u, dim3(g2), dim3(RT), 0, s, (const double *)(WT + (size_t)k * S), (const double *)hd,
                            dev, BN);
-        if (keep_ck) {  // the forward's W factors, checkpoints, Q and phi of u_k
+        if (keep_ck) {  // the forward's W factors, checkpoints (and opened states), Q and phi of u_k
             HIPX(ctx, ctx->fp32 ? enqueue_prep_scratch<float>(ctx) : enqueue_prep_scratch<double>(ctx));
-            HIPX(ctx, hipMemcpyAsync(ctx->slots, CK + (size_t)k * ck_bytes, ck_bytes, hipMemcpyDeviceToDevice, s));
+            HIPX(ctx, hipMemcpyAsync(ctx->slots, CK + (size_t)k * ck_rec, ck_bytes, hipMemcpyDeviceToDevice, s));
+            if (keep_open)
+                HIPX(ctx, hipMemcpyAsync(open_slots, CK + (size_t)k * ck_rec + ck_bytes, ck_bytes, hipMemcpyDeviceToDevice,
+                                         s));
             HIPX(ctx, hipMemcpyAsync(ctx->Q, Qk + (size_t)k * nb, 8 * nb, hipMemcpyDeviceToDevice, s));
             HIPX(ctx, hipMemcpyAsync(dev + 6 * BN, PH + (size_t)k * S, 8 * S, hipMemcpyDeviceToDevice, s));
         }

@@ -37,8 +37,10 @@ def test_checkpoint_conc_matches_full(pkg, shape, fA, ds, prec):
             s.set_fields(wA, wB)
             Q = s.concentrations()[0]
             phiA, phiB, _, _ = s.get_fields()
-            if ck:
-                assert s.device_bytes < out[0][3]  # fewer slices held
+            if ck and ds <= 0.02:
+                # fewer slices held (for the 20-step chain the checkpoints, their
+                # opened states and the working slices outnumber its 23 slices)
+                assert s.device_bytes < out[0][3]
             out.append((Q, phiA, phiB, s.device_bytes))
     tol = 1e-13 if prec == 0 else 1e-6
     assert abs(out[0][0] - out[1][0]) / out[0][0] < tol
@@ -217,14 +219,15 @@ def test_adjoint_loop_batch_matches_single(pkg, mixing):
 @pytest.mark.parametrize("mixing,prec", [(0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (1, 2)])
 def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     """sqrt(Ns) checkpointing in the loop adjoint (the reverse sweep restores
-    each iteration's checkpoints instead of re-running its forward march)
-    gives the full store's loss and gradient.  The forward evaluations
-    re-march from the march's opened states, so the loss is the full store's
-    bit for bit; the reverse sweep's pair walk re-marches from checkpoint
-    slices, whose first launch (a plain kernel where the march has a boundary
-    kernel) rounds differently in FP32 storage: measured gradient differences
-    2.6e-7 / 3.2e-6 (FP32, simple / Anderson) and 6.6e-8 / 1.1e-6 (FP32X),
-    bound 2e-5."""
+    each iteration's checkpoints and opened states instead of re-running its
+    forward march) gives the full store's loss and gradient.  The forward
+    evaluations and the reverse sweep's pair walk re-march from the march's
+    opened states, so in the FP32-storage forms the loss is the full store's
+    bit for bit, and so is the gradient through simple mixing.  Through
+    Anderson mixing the gradients are equal for up to three iterations; from
+    the fourth (m = 4) a difference of 2e-15 appears, which the Anderson loop
+    amplifies through the FP32 rounding of the W factors to 3.0e-6 (FP32) and
+    7.5e-7 (FP32X) after 8 iterations: bound 1e-5."""
     res = {}
     for ck in (0, 1):
         p, N, h, w0A, w0B, target, iters = _loop_case(pkg, mixing, prec=prec, checkpoint=ck)
@@ -236,7 +239,7 @@ def test_adjoint_loop_checkpoint_matches_full(pkg, mixing, prec):
     if prec == 0:
         assert dl < 1e-12 and dg < 1e-10
     else:
-        assert dl == 0.0 and dg < 2e-5
+        assert dl == 0.0 and (dg == 0.0 if mixing == 0 else dg < 1e-5)
 
 
 def test_adjoint_loop_deterministic(pkg):
