@@ -254,11 +254,13 @@ def test_q_underflow_reported(pkg):
 @pytest.mark.parametrize("shape,prec", [((32, 32, 32), 0), ((64, 64, 8), 0), ((48, 48, 6), 1),
                                         ((128, 128, 4), 1), ((128, 128, 4), 0), ((64, 64, 1), 0),
                                         ((16, 16, 24), 1), ((16, 16, 24), 0), ((64, 64, 8), 2), ((128, 128, 4), 2),
-                                        ((48, 48, 6), 2)])
+                                        ((48, 48, 6), 2), ((48, 48, 6), 0)])
 def test_xy_plane_kernel(pkg, orc, shape, prec):
     """Every xy-plane kernel form (1: one workgroup per plane; 2, 3: split into
     the real and imaginary parts, two- and three-level tiling; 4, 5: the split
-    forms with lane-local y lines) equals the separate-pass march and the oracle."""
+    forms with lane-local y lines) equals the separate-pass march and the oracle.
+    The split kernels of 48 and 16 points run both parts of a plane in one
+    workgroup."""
     print("case", shape, prec, flush=True)
     out = {}
     # the FP64 128^2 plane fits LDS only in the two-level split form
