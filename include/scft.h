@@ -188,7 +188,8 @@ int scft_conc_adjoint(scft_ctx *ctx, const double *uA, const double *uB, const d
    phiA_last (may be NULL) receives the last phiA.  The trajectory and its
    adjoint stay in device memory: 2 * batch * N doubles per iteration for
    each of 3 arrays (5 with grad_h; in checkpoint mode with grad_h also the
-   iteration's checkpoints, about sqrt(Ns) propagator slices), held by the
+   iteration's checkpoints and the march's opened states beside them, about
+   2 sqrt(Ns) propagator slices), held by the
    context after the call and grown when a later call needs more; only the
    m x m least-squares solves run on the host. */
 int scft_adjoint_loop(scft_ctx *ctx, const double *h, const double *w0A, const double *w0B, int n_iter,

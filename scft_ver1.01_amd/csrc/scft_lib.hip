@@ -1907,7 +1907,7 @@ struct PairWalk {
 // Slots: checkpoint k at k; window z(JK + 1 + i) and the re-marched z(jK + i)
 // at nck + 1 + i and nck + i; the march's z(Ns - jK - K + 1 + i) at nck + K + i;
 // each chunk's last slice in the carry pair nck + 2K, nck + 2K + 1; the steps
-// before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3; O(jK)
+// before the window in the ping-pong pair nck + 2K + 2, nck + 2K + 3; O(kK)
 // at nck + 2K + 4 + k and the re-marches' work field at nck + 2K + 4 + nck.
 template <typename T>
 static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *phiB)
