@@ -941,7 +941,7 @@ struct scft_ctx {
     int and_maxm = 0;         // accumulator bucket (8, 16 or 32) >= anderson_m
     // one SCFT iteration captured as a HIP graph, and two in a row (built on
     // first use): runs of iterations replay the pair graph, which saves one
-    // graph launch boundary per pair (measured 8 us per launch at 64^3)
+    // graph launch boundary per pair (64^3 FP32 iteration 1.592 -> 1.588 ms)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph2 = nullptr;
