@@ -295,71 +295,87 @@ __device__ __forceinline__ ZGroup<T, P> ld_group(const RGroup<T, P> *re, long im
 // Ns - s are loaded once for both products.  P consecutive points per thread
 // (one 16-byte load per slice: P = 2 in FP32, 1 in FP64; N % P == 0), 4 slice
 // pairs in flight.  Each point's sum runs in the order of the reference's
-// s-loop restricted to the pairs, the same for every P.
+// s-loop restricted to the pairs, the same for every P.  Every block then
+// finishes Q = sum/N of k_qsum's block partials qpart (the sum k_q_final
+// forms, in the same order) for its factor ds/Q, and block 0 stores Q and
+// ds/Q: one launch less per evaluation.
 template <typename T, int P, int K = 4>
 __global__ void __launch_bounds__(RT) k_contour(const typename cplx<T>::type *__restrict__ slots, long N, int batch,
-                                                const int2 *__restrict__ nsab, const double *__restrict__ ffl,
-                                                double *__restrict__ phiA, double *__restrict__ phiB)
+                                                const int2 *__restrict__ nsab, const double *__restrict__ qpart,
+                                                int nblk, double ds, double *__restrict__ Qout,
+                                                double *__restrict__ fflout, double *__restrict__ phiA,
+                                                double *__restrict__ phiB)
 {
     using G = ZGroup<T, P>;
     using RG = RGroup<T, P>;
+    __shared__ double sh[2 * RT / 64 + 2];
     const int b = blockIdx.y;
     const int NsA = nsab[b].x, Ns = nsab[b].x + nsab[b].y;
     const long g = (long)blockIdx.x * RT + threadIdx.x;
-    if (g * P >= N) return;
+    const bool act = g * P < N;  // the whole block takes part in the Q sum below
     const long BNg = (long)batch * N / P;  // one plane of a slice, in groups
     const RG *zr = reinterpret_cast<const RG *>(slots) + (b * N) / P + g;
     auto zp_at = [&](long s) { return ld_group<T, P>(zr + s * 2 * BNg, BNg, 0); };
     double accA[P], accB[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) accA[p] = accB[p] = 0.0;
-    const int half = Ns / 2;  // pairs s = 0..half (s == Ns - s only when Ns is even and s == half)
-    int s = 0;
-    for (; s + K - 1 <= half; s += K) {
-        G zs[K], zu[K];
+    if (act) {
+        const int half = Ns / 2;  // pairs s = 0..half (s == Ns - s only when Ns is even and s == half)
+        int s = 0;
+        for (; s + K - 1 <= half; s += K) {
+            G zs[K], zu[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            zs[k] = zp_at(s + k);
-            zu[k] = zp_at(Ns - s - k);
+            for (int k = 0; k < K; ++k) {
+                zs[k] = zp_at(s + k);
+                zu[k] = zp_at(Ns - s - k);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int s1 = s + k, u1 = Ns - s - k;
+                double wa1, wb1, wa2, wb2;
+                trap_w(s1, NsA, Ns, wa1, wb1);
+                trap_w(u1, NsA, Ns, wa2, wb2);
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const double p1 = (double)zs[k].re.v[p] * (double)zu[k].im.v[p];  // q(s1) qdag(s1)
+                    accA[p] += wa1 * p1;
+                    accB[p] += wb1 * p1;
+                    if (u1 != s1) {
+                        const double p2 = (double)zu[k].re.v[p] * (double)zs[k].im.v[p];  // q(u1) qdag(u1)
+                        accA[p] += wa2 * p2;
+                        accB[p] += wb2 * p2;
+                    }
+                }
+            }
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int s1 = s + k, u1 = Ns - s - k;
+        for (; s <= half; ++s) {
+            const int u1 = Ns - s;
+            const G a = zp_at(s), c = zp_at(u1);
             double wa1, wb1, wa2, wb2;
-            trap_w(s1, NsA, Ns, wa1, wb1);
+            trap_w(s, NsA, Ns, wa1, wb1);
             trap_w(u1, NsA, Ns, wa2, wb2);
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                const double p1 = (double)zs[k].re.v[p] * (double)zu[k].im.v[p];  // q(s1) qdag(s1)
+                const double p1 = (double)a.re.v[p] * (double)c.im.v[p];
                 accA[p] += wa1 * p1;
                 accB[p] += wb1 * p1;
-                if (u1 != s1) {
-                    const double p2 = (double)zu[k].re.v[p] * (double)zs[k].im.v[p];  // q(u1) qdag(u1)
+                if (u1 != s) {
+                    const double p2 = (double)c.re.v[p] * (double)a.im.v[p];
                     accA[p] += wa2 * p2;
                     accB[p] += wb2 * p2;
                 }
             }
         }
     }
-    for (; s <= half; ++s) {
-        const int u1 = Ns - s;
-        const G a = zp_at(s), c = zp_at(u1);
-        double wa1, wb1, wa2, wb2;
-        trap_w(s, NsA, Ns, wa1, wb1);
-        trap_w(u1, NsA, Ns, wa2, wb2);
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const double p1 = (double)a.re.v[p] * (double)c.im.v[p];
-            accA[p] += wa1 * p1;
-            accB[p] += wb1 * p1;
-            if (u1 != s) {
-                const double p2 = (double)c.re.v[p] * (double)a.im.v[p];
-                accA[p] += wa2 * p2;
-                accB[p] += wb2 * p2;
-            }
-        }
+    double qv[1] = {0.0};
+    for (int k = threadIdx.x; k < nblk; k += RT) qv[0] += qpart[(b * 2) * nblk + k];
+    block_sums<1>(qv, sh);
+    const double Q = qv[0] / N, f = ds / Q;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Qout[b] = Q;
+        fflout[b] = f;
     }
-    const double f = ffl[b];
+    if (!act) return;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         phiA[b * N + g * P + p] = accA[p] * f;
@@ -2025,22 +2041,26 @@ static hipError_t enqueue_march_contour_ck(scft_ctx *c, double *phiA, double *ph
 // The trapezoid contour over the full slice store (k_contour): P consecutive
 // points per thread, so that each plane of a slice moves in 16-byte loads (4
 // points in FP32, 2 in FP64, fewer when N is not a multiple); 4 pairs of
-// slices in flight per thread.
+// slices in flight per thread.  It also finishes Q from k_qsum's partials
+// (part2) into Qout and ds/Q into fflout.
 template <typename T>
-static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB)
+static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB, double *Qout, double *fflout)
 {
     using C = typename cplx<T>::type;
     const int P = contour_points<T>(c->N);
     const dim3 gc((unsigned)((c->N / P + RT - 1) / RT), c->batch);
     if (P == 4)
         hipLaunchKernelGGL((k_contour<T, 4>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+                           (const int2 *)c->nsab, (const double *)c->part2, c->nblk, c->p.ds, Qout, fflout, phiA,
+                           phiB);
     else if (P == 2)
         hipLaunchKernelGGL((k_contour<T, 2>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+                           (const int2 *)c->nsab, (const double *)c->part2, c->nblk, c->p.ds, Qout, fflout, phiA,
+                           phiB);
     else
         hipLaunchKernelGGL((k_contour<T, 1>), gc, dim3(RT), 0, c->stream, (const C *)c->slots, c->N, c->batch,
-                           (const int2 *)c->nsab, c->ffl, phiA, phiB);
+                           (const int2 *)c->nsab, (const double *)c->part2, c->nblk, c->p.ds, Qout, fflout, phiA,
+                           phiB);
     return hipGetLastError();
 }
 
@@ -2062,9 +2082,7 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
     // full store: each instance's q(Ns_b) is slice Ns_b
     hipLaunchKernelGGL(k_qsum<T>, g, dim3(RT), 0, c->stream, (const C *)zNs, (const C *)c->slots,
                        (const int2 *)c->nsab, c->N, c->batch, c->nblk, c->part2);
-    hipLaunchKernelGGL(k_q_final, dim3(c->batch), dim3(RT), 0, c->stream, c->part2, c->nblk, c->N, c->p.ds, c->Q,
-                       c->ffl);
-    return launch_contour<T>(c, phiA, phiB);
+    return launch_contour<T>(c, phiA, phiB, c->Q, c->ffl);  // Q and ds/Q too
 }
 
 template <typename T>
@@ -2769,7 +2787,7 @@ static int ensure_adjoint_scratch(scft_ctx *c)
 }
 
 // `reps` contour launches over the current slice store into the adjoint
-// scratch (phi and Q of the context are not touched)
+// scratch (phi, Q and ds/Q of the context are not touched)
 static int time_contour(scft_ctx *ctx, int reps, float *ms)
 {
     if (ctx->ckpt)
@@ -2778,8 +2796,10 @@ static int time_contour(scft_ctx *ctx, int reps, float *ms)
     int rc = ensure_adjoint_scratch(ctx);
     if (rc) return rc;
     const size_t BN = (size_t)ctx->batch * ctx->N;
-    double *pA = ctx->adj + 6 * BN, *pB = pA + BN;
-    auto one = [&]() { return ctx->fp32 ? launch_contour<float>(ctx, pA, pB) : launch_contour<double>(ctx, pA, pB); };
+    double *pA = ctx->adj + 6 * BN, *pB = pA + BN, *q = pB + BN, *f = q + ctx->batch;
+    auto one = [&]() {
+        return ctx->fp32 ? launch_contour<float>(ctx, pA, pB, q, f) : launch_contour<double>(ctx, pA, pB, q, f);
+    };
     HIPX(ctx, one());  // untimed
     HIPX(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int r = 0; r < reps; ++r) HIPX(ctx, one());
