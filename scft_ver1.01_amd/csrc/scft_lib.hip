@@ -152,10 +152,10 @@ __global__ void __launch_bounds__(RT) k_center(double *__restrict__ f0, double *
 // and W_A*post, W_B*post (after the last pass), post = 1/N times the factor of
 // the untransformed length-1 axes -- formed in FP64 and rounded once, so the
 // k = 0 mode of every pass is multiplied by exactly 1.
-template <typename T>
+template <typename T, typename WT>
 __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const double *__restrict__ wB, long N, int nblk,
                                              const double *__restrict__ part, double ds2,
-                                             const double *__restrict__ post_b, T *__restrict__ wdz,
+                                             const double *__restrict__ post_b, WT *__restrict__ wdz,
                                              typename cplx<T>::type *__restrict__ slot0, int batch)
 {
     __shared__ double sh[2 * RT / 64 + 2];
@@ -171,10 +171,10 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
             wA[i] = a;
         }
         const double ea = exp(-a * ds2), eb = exp(-wB[i] * ds2);
-        wdz[i] = (T)ea;
-        wdz[BN + i] = (T)eb;
-        wdz[2 * BN + i] = (T)(ea * post);
-        wdz[3 * BN + i] = (T)(eb * post);
+        wdz[i] = (WT)ea;
+        wdz[BN + i] = (WT)eb;
+        wdz[2 * BN + i] = (WT)(ea * post);
+        wdz[3 * BN + i] = (WT)(eb * post);
         if (slot0) {  // planar: real parts, then imaginary parts
             reinterpret_cast<T *>(slot0)[i] = T(1);
             reinterpret_cast<T *>(slot0)[BN + i] = T(1);
@@ -937,9 +937,10 @@ struct scft_ctx {
     int nblk = 1;
     size_t csize = 16;  // bytes per complex element
     size_t tsize = 8;   // bytes per real element
+    size_t wsize = 8;   // bytes per W factor: 4 in SCFT_FP32, 8 in SCFT_FP64 and SCFT_FP32X
     // device buffers
     double *wA = nullptr, *wB = nullptr, *phiA = nullptr, *phiB = nullptr;
-    void *wdz = nullptr;    // T [4][batch][N]: W_A, W_B, W_A*post, W_B*post
+    void *wdz = nullptr;    // [4][batch][N] of wsize bytes: W_A, W_B, W_A*post, W_B*post
     double post = 1.0;      // 1/N times the factor of the length-1 axes
     void *slots = nullptr;  // C [Ns+1][batch][N]
     void *work = nullptr;   // C [batch][N]
@@ -1319,11 +1320,12 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     c->mixed = p.precision == SCFT_FP32X;
     c->csize = c->fp32 ? 8 : 16;
     c->tsize = c->fp32 ? 4 : 8;
+    c->wsize = (c->fp32 && !c->mixed) ? 4 : 8;
     // the pass kernels address the four W arrays ([4][batch][N]) with 32-bit byte offsets
-    if ((double)4 * c->tsize * c->batch * c->N >= 4294967296.0)
+    if ((double)4 * c->wsize * c->batch * c->N >= 4294967296.0)
         return set_err(nullptr, SCFT_E_UNSUPPORTED,
-                       "batch * N too large for one context (4 * batch * N * sizeof(real) must stay below 2^32 "
-                       "bytes): split the batch over several contexts");
+                       "batch * N too large for one context (4 * batch * N * sizeof(W factor) must stay below 2^32 "
+                       "bytes; the W factors are FP64 except in SCFT_FP32): split the batch over several contexts");
     for (int a = 0; a < 3; ++a) {
         c->n[a] = n[a];
         c->active[a] = n[a] > 1;
@@ -1384,7 +1386,8 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     A((void **)&c->wB, 8 * BN);
     A((void **)&c->phiA, 8 * BN);
     A((void **)&c->phiB, 8 * BN);
-    A(&c->wdz, 4 * c->tsize * BN);
+    // FP64-sized where scft_set_precision may switch an FP32 context to FP32X
+    A(&c->wdz, 4 * (c->wsize == 4 && (double)32 * BN >= 4294967296.0 ? 4 : 8) * BN);
     c->ckpt = p.checkpoint ? 1 : 0;
     const long nslots = slot_count(c->Ns_max, c->ckpt, &c->ck_K, &c->ck_n);
     A((void **)&c->nsab, sizeof(int2) * c->batch);
@@ -1637,7 +1640,7 @@ static PassArgs<T> pass_args(scft_ctx *c, int axis, const typename cplx<T>::type
     PassArgs<T> pa;
     pa.src = src;
     pa.dst = dst;
-    pa.W = (const T *)c->wdz;  // [pre A, pre B, post A, post B]
+    pa.W = c->wdz;  // [pre A, pre B, post A, post B]
     pa.wo[0] = (unsigned)(br * BN);
     pa.wo[1] = (unsigned)(bi * BN);
     pa.wo[2] = (unsigned)((2 + br) * BN);
@@ -2064,6 +2067,19 @@ static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB, double
     return hipGetLastError();
 }
 
+// k_prep with the context's W factor type (FP32 only in SCFT_FP32)
+template <typename T>
+static void launch_prep(scft_ctx *c, double *fA, const double *fB, const double *mean_part)
+{
+    const dim3 g(c->nblk, c->batch);
+    if (sizeof(T) == 4 && c->wsize == 4)
+        hipLaunchKernelGGL((k_prep<T, float>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
+                           c->p.ds / 2, (const double *)c->post_b, (float *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+    else
+        hipLaunchKernelGGL((k_prep<T, double>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
+                           c->p.ds / 2, (const double *)c->post_b, (double *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+}
+
 // getConc on the device for fields (fA, fB); mean_part: partial sums of fA to
 // subtract first (or null: fields used as given)
 template <typename T>
@@ -2072,8 +2088,7 @@ static hipError_t enqueue_conc_fields(scft_ctx *c, double *fA, const double *fB,
 {
     using C = typename cplx<T>::type;
     const dim3 g(c->nblk, c->batch);
-    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part, c->p.ds / 2,
-                       (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+    launch_prep<T>(c, fA, fB, mean_part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (c->ckpt) return enqueue_march_contour_ck<T>(c, phiA, phiB);  // one Ns for all instances
@@ -2214,9 +2229,9 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
     const size_t BN = (size_t)ctx->batch * ctx->N;
     const double *w = block == 0 ? ctx->wA : ctx->wB;
     double *stage = (double *)ctx->work;  // BN doubles fit in the complex work buffer
-    char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->tsize;
-    char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->tsize;
-    if (ctx->fp32)
+    char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->wsize;
+    char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->wsize;
+    if (ctx->wsize == 4)
         hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN, ctx->N,
                            ctx->p.ds / 2, (const double *)ctx->post_b, (float *)wdz_b, (float *)wpost_b);
     else
@@ -2462,7 +2477,7 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     PassArgs<T> pa;
     pa.src = (const C *)c->work;
     pa.dst = (C *)c->work;
-    pa.W = (const T *)c->wdz;
+    pa.W = c->wdz;
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
     // codes: see kPlaneCode.  A boundary kernel runs in place on the work
     // field, as in the march, with the arguments of consecutive contour steps.
@@ -2519,21 +2534,23 @@ static int time_march(scft_ctx *ctx, int reps, float *ms)
 {
     const size_t BN = (size_t)ctx->batch * ctx->N;
     const double invN = 1.0 / (double)ctx->N;
-    if (ctx->fp32) {
+    if (ctx->wsize == 4) {
         hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz, 1.0f,
                            (long)(2 * BN));
         hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz + 2 * BN,
                            (float)invN, (long)(2 * BN));
-        hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
-                           (const double *)nullptr, (long)BN);
     } else {
         hipLaunchKernelGGL(k_fill<double>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (double *)ctx->wdz, 1.0,
                            (long)(2 * BN));
         hipLaunchKernelGGL(k_fill<double>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream,
                            (double *)ctx->wdz + 2 * BN, invN, (long)(2 * BN));
+    }
+    if (ctx->fp32)
+        hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
+                           (const double *)nullptr, (long)BN);
+    else
         hipLaunchKernelGGL(k_seed_real<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (double2 *)ctx->slots,
                            (const double *)nullptr, (long)BN);
-    }
     HIPX(ctx, hipGetLastError());
     auto march = [&]() {
         return ctx->fp32 ? enqueue_march<float>(ctx, ctx->Ns_max, -1) : enqueue_march<double>(ctx, ctx->Ns_max, -1);
@@ -2741,6 +2758,9 @@ extern "C" int scft_set_precision(scft_ctx *ctx, int precision)
         return set_err(ctx, SCFT_E_STATE, "scft_set_precision needs a context created with FP32 storage "
                                           "(SCFT_FP32 or SCFT_FP32X)");
     if (precision == ctx->p.precision) return SCFT_OK;
+    if (precision == SCFT_FP32X && (double)32 * ctx->batch * ctx->N >= 4294967296.0)
+        return set_err(ctx, SCFT_E_UNSUPPORTED, "SCFT_FP32X keeps FP64 W factors: 4 * batch * N * 8 bytes must stay "
+                                                "below 2^32");
     HIPX(ctx, hipSetDevice(ctx->device));
     HIPX(ctx, hipStreamSynchronize(ctx->stream));
     for (int a = 0; a < 3; ++a) {
@@ -2748,6 +2768,7 @@ extern "C" int scft_set_precision(scft_ctx *ctx, int precision)
         std::swap(ctx->kf[a], ctx->kf_alt[a]);
     }
     ctx->mixed = precision == SCFT_FP32X;
+    ctx->wsize = ctx->mixed ? 8 : 4;
     ctx->p.precision = precision;
     // the tiling carries over where the new form has it (its registers and LDS differ); else the
     // default variant and separate x and y passes
@@ -2815,11 +2836,8 @@ static int time_contour(scft_ctx *ctx, int reps, float *ms)
 template <typename T>
 static hipError_t enqueue_prep_scratch(scft_ctx *c)
 {
-    const dim3 g(c->nblk, c->batch);
     double *uA = c->adj, *uB = uA + (size_t)c->batch * c->N;
-    hipLaunchKernelGGL(k_prep<T>, g, dim3(RT), 0, c->stream, uA, (const double *)uB, c->N, c->nblk,
-                       (const double *)nullptr, c->p.ds / 2, (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0),
-                       c->batch);
+    launch_prep<T>(c, uA, (const double *)uB, (const double *)nullptr);
     return hipGetLastError();
 }
 
@@ -3837,7 +3855,7 @@ static int propagate_noalloc(scft_ctx &c, unsigned char *scr, int block, int sig
     if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("table upload: ") + hipGetErrorString(e));
     size_t p = NA_TABLE_BYTES;
     c.wdz = scr + p;
-    p = align256(p + 4 * BN * sizeof(T));
+    p = align256(p + 4 * BN * c.wsize);
     C *sl[2] = {(C *)(scr + p), nullptr};
     p = align256(p + BN * sizeof(C));
     sl[1] = (C *)(scr + p);
@@ -3846,11 +3864,16 @@ static int propagate_noalloc(scft_ctx &c, unsigned char *scr, int block, int sig
     p = align256(p + BN * sizeof(C));
     if (p > NA_SCRATCH_BYTES) return set_err(nullptr, SCFT_E_INVALID, "no-allocation propagate: scratch overflow");
     c.slots = sl[0];
-    T *wpre = (T *)c.wdz + (size_t)block * BN, *wpost = (T *)c.wdz + (size_t)(2 + block) * BN;
     const int ns = block == 0 ? c.NsA : c.NsB;
     const long stride = (long)(ns + 1) * c.N;
-    hipLaunchKernelGGL(k_wdz_one<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
-                       (const double *)c.post_b, wpre, wpost);
+    if (c.wsize == 4)
+        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
+                           (const double *)c.post_b, (float *)c.wdz + (size_t)block * BN,
+                           (float *)c.wdz + (size_t)(2 + block) * BN);
+    else
+        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
+                           (const double *)c.post_b, (double *)c.wdz + (size_t)block * BN,
+                           (double *)c.wdz + (size_t)(2 + block) * BN);
     hipLaunchKernelGGL(k_seed_real<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0], q_init, (long)BN);
     hipLaunchKernelGGL(k_take_real_to<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0],
                        q_out + (sign == 1 ? 0 : (long)ns * c.N), (long)BN, c.N, stride);
@@ -3921,6 +3944,7 @@ extern "C" int scft_propagate_noalloc(const scft_params *pin, int device, int bl
     c.mixed = p.precision == SCFT_FP32X;
     c.csize = c.fp32 ? 8 : 16;
     c.tsize = c.fp32 ? 4 : 8;
+    c.wsize = (c.fp32 && !c.mixed) ? 4 : 8;
     for (int a = 0; a < 3; ++a) {
         c.n[a] = n[a];
         c.active[a] = n[a] > 1;
