@@ -152,10 +152,10 @@ __global__ void __launch_bounds__(RT) k_center(double *__restrict__ f0, double *
 // and W_A*post, W_B*post (after the last pass), post = 1/N times the factor of
 // the untransformed length-1 axes -- formed in FP64 and rounded once, so the
 // k = 0 mode of every pass is multiplied by exactly 1.
-template <typename T, typename WT>
+template <typename T, bool DELTA>
 __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const double *__restrict__ wB, long N, int nblk,
                                              const double *__restrict__ part, double ds2,
-                                             const double *__restrict__ post_b, WT *__restrict__ wdz,
+                                             const double *__restrict__ post_b, T *__restrict__ wdz,
                                              typename cplx<T>::type *__restrict__ slot0, int batch)
 {
     __shared__ double sh[2 * RT / 64 + 2];
@@ -171,10 +171,10 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
             wA[i] = a;
         }
         const double ea = exp(-a * ds2), eb = exp(-wB[i] * ds2);
-        wdz[i] = (WT)ea;
-        wdz[BN + i] = (WT)eb;
-        wdz[2 * BN + i] = (WT)(ea * post);
-        wdz[3 * BN + i] = (WT)(eb * post);
+        wdz[i] = DELTA ? (T)expm1(-a * ds2) : (T)ea;  // DELTA (FP32X): the pre factors as W - 1
+        wdz[BN + i] = DELTA ? (T)expm1(-wB[i] * ds2) : (T)eb;
+        wdz[2 * BN + i] = (T)(ea * post);
+        wdz[3 * BN + i] = (T)(eb * post);
         if (slot0) {  // planar: real parts, then imaginary parts
             reinterpret_cast<T *>(slot0)[i] = T(1);
             reinterpret_cast<T *>(slot0)[BN + i] = T(1);
@@ -183,14 +183,14 @@ __global__ void __launch_bounds__(RT) k_prep(double *__restrict__ wA, const doub
 }
 
 // W (pre) and W*post of one block only (sovDifFft computes W from the field as given)
-template <typename T>
+template <typename T, bool DELTA>
 __global__ void __launch_bounds__(RT) k_wdz_one(const double *__restrict__ w, long n, long N, double ds2,
                                                 const double *__restrict__ post_b, T *__restrict__ wpre,
                                                 T *__restrict__ wpost)
 {
     for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) {
         const double e = exp(-w[i] * ds2);
-        wpre[i] = (T)e;
+        wpre[i] = DELTA ? (T)expm1(-w[i] * ds2) : (T)e;  // DELTA (FP32X): the pre factor as W - 1
         wpost[i] = (T)(e * post_b[i / N]);
     }
 }
@@ -937,10 +937,9 @@ struct scft_ctx {
     int nblk = 1;
     size_t csize = 16;  // bytes per complex element
     size_t tsize = 8;   // bytes per real element
-    size_t wsize = 8;   // bytes per W factor: 4 in SCFT_FP32, 8 in SCFT_FP64 and SCFT_FP32X
     // device buffers
     double *wA = nullptr, *wB = nullptr, *phiA = nullptr, *phiB = nullptr;
-    void *wdz = nullptr;    // [4][batch][N] of wsize bytes: W_A, W_B, W_A*post, W_B*post
+    void *wdz = nullptr;    // T [4][batch][N]: W_A, W_B, W_A*post, W_B*post (FP32X: W_A - 1, W_B - 1, ...)
     double post = 1.0;      // 1/N times the factor of the length-1 axes
     void *slots = nullptr;  // C [Ns+1][batch][N]
     void *work = nullptr;   // C [batch][N]
@@ -1320,12 +1319,11 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     c->mixed = p.precision == SCFT_FP32X;
     c->csize = c->fp32 ? 8 : 16;
     c->tsize = c->fp32 ? 4 : 8;
-    c->wsize = (c->fp32 && !c->mixed) ? 4 : 8;
     // the pass kernels address the four W arrays ([4][batch][N]) with 32-bit byte offsets
-    if ((double)4 * c->wsize * c->batch * c->N >= 4294967296.0)
+    if ((double)4 * c->tsize * c->batch * c->N >= 4294967296.0)
         return set_err(nullptr, SCFT_E_UNSUPPORTED,
-                       "batch * N too large for one context (4 * batch * N * sizeof(W factor) must stay below 2^32 "
-                       "bytes; the W factors are FP64 except in SCFT_FP32): split the batch over several contexts");
+                       "batch * N too large for one context (4 * batch * N * sizeof(real) must stay below 2^32 "
+                       "bytes): split the batch over several contexts");
     for (int a = 0; a < 3; ++a) {
         c->n[a] = n[a];
         c->active[a] = n[a] > 1;
@@ -1386,8 +1384,7 @@ static int create_ctx(const scft_params *pin, const double *inst_L, const double
     A((void **)&c->wB, 8 * BN);
     A((void **)&c->phiA, 8 * BN);
     A((void **)&c->phiB, 8 * BN);
-    // FP64-sized where scft_set_precision may switch an FP32 context to FP32X
-    A(&c->wdz, 4 * (c->wsize == 4 && (double)32 * BN >= 4294967296.0 ? 4 : 8) * BN);
+    A(&c->wdz, 4 * c->tsize * BN);
     c->ckpt = p.checkpoint ? 1 : 0;
     const long nslots = slot_count(c->Ns_max, c->ckpt, &c->ck_K, &c->ck_n);
     A((void **)&c->nsab, sizeof(int2) * c->batch);
@@ -2067,17 +2064,17 @@ static hipError_t launch_contour(scft_ctx *c, double *phiA, double *phiB, double
     return hipGetLastError();
 }
 
-// k_prep with the context's W factor type (FP32 only in SCFT_FP32)
+// k_prep with the context's W factor form (SCFT_FP32X: pre factors as W - 1)
 template <typename T>
 static void launch_prep(scft_ctx *c, double *fA, const double *fB, const double *mean_part)
 {
     const dim3 g(c->nblk, c->batch);
-    if (sizeof(T) == 4 && c->wsize == 4)
-        hipLaunchKernelGGL((k_prep<T, float>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
-                           c->p.ds / 2, (const double *)c->post_b, (float *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+    if (c->mixed)
+        hipLaunchKernelGGL((k_prep<T, true>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
+                           c->p.ds / 2, (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
     else
-        hipLaunchKernelGGL((k_prep<T, double>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
-                           c->p.ds / 2, (const double *)c->post_b, (double *)c->wdz, slot_ptr<T>(c, 0), c->batch);
+        hipLaunchKernelGGL((k_prep<T, false>), g, dim3(RT), 0, c->stream, fA, fB, c->N, c->nblk, mean_part,
+                           c->p.ds / 2, (const double *)c->post_b, (T *)c->wdz, slot_ptr<T>(c, 0), c->batch);
 }
 
 // getConc on the device for fields (fA, fB); mean_part: partial sums of fA to
@@ -2229,14 +2226,17 @@ extern "C" int scft_propagate(scft_ctx *ctx, int block, int sign, const double *
     const size_t BN = (size_t)ctx->batch * ctx->N;
     const double *w = block == 0 ? ctx->wA : ctx->wB;
     double *stage = (double *)ctx->work;  // BN doubles fit in the complex work buffer
-    char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->wsize;
-    char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->wsize;
-    if (ctx->wsize == 4)
-        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN, ctx->N,
-                           ctx->p.ds / 2, (const double *)ctx->post_b, (float *)wdz_b, (float *)wpost_b);
+    char *wdz_b = (char *)ctx->wdz + (size_t)block * BN * ctx->tsize;
+    char *wpost_b = (char *)ctx->wdz + (size_t)(2 + block) * BN * ctx->tsize;
+    if (ctx->mixed)
+        hipLaunchKernelGGL((k_wdz_one<float, true>), dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
+                           ctx->N, ctx->p.ds / 2, (const double *)ctx->post_b, (float *)wdz_b, (float *)wpost_b);
+    else if (ctx->fp32)
+        hipLaunchKernelGGL((k_wdz_one<float, false>), dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
+                           ctx->N, ctx->p.ds / 2, (const double *)ctx->post_b, (float *)wdz_b, (float *)wpost_b);
     else
-        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN, ctx->N,
-                           ctx->p.ds / 2, (const double *)ctx->post_b, (double *)wdz_b, (double *)wpost_b);
+        hipLaunchKernelGGL((k_wdz_one<double, false>), dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, w, (long)BN,
+                           ctx->N, ctx->p.ds / 2, (const double *)ctx->post_b, (double *)wdz_b, (double *)wpost_b);
     if (q_init) HIPX(ctx, hipMemcpyAsync(stage, q_init, 8 * BN, hipMemcpyHostToDevice, ctx->stream));
     if (ctx->fp32)
         hipLaunchKernelGGL(k_seed_real<float>, dim3(pt_grid(BN)), dim3(RT), 0, ctx->stream, (float2 *)ctx->slots,
@@ -2534,9 +2534,9 @@ static int time_march(scft_ctx *ctx, int reps, float *ms)
 {
     const size_t BN = (size_t)ctx->batch * ctx->N;
     const double invN = 1.0 / (double)ctx->N;
-    if (ctx->wsize == 4) {
-        hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz, 1.0f,
-                           (long)(2 * BN));
+    if (ctx->fp32) {  // pre factors 1 (stored as 0 in FP32X, where they are W - 1)
+        hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz,
+                           ctx->mixed ? 0.0f : 1.0f, (long)(2 * BN));
         hipLaunchKernelGGL(k_fill<float>, dim3(pt_grid(2 * BN)), dim3(RT), 0, ctx->stream, (float *)ctx->wdz + 2 * BN,
                            (float)invN, (long)(2 * BN));
     } else {
@@ -2758,9 +2758,6 @@ extern "C" int scft_set_precision(scft_ctx *ctx, int precision)
         return set_err(ctx, SCFT_E_STATE, "scft_set_precision needs a context created with FP32 storage "
                                           "(SCFT_FP32 or SCFT_FP32X)");
     if (precision == ctx->p.precision) return SCFT_OK;
-    if (precision == SCFT_FP32X && (double)32 * ctx->batch * ctx->N >= 4294967296.0)
-        return set_err(ctx, SCFT_E_UNSUPPORTED, "SCFT_FP32X keeps FP64 W factors: 4 * batch * N * 8 bytes must stay "
-                                                "below 2^32");
     HIPX(ctx, hipSetDevice(ctx->device));
     HIPX(ctx, hipStreamSynchronize(ctx->stream));
     for (int a = 0; a < 3; ++a) {
@@ -2768,7 +2765,6 @@ extern "C" int scft_set_precision(scft_ctx *ctx, int precision)
         std::swap(ctx->kf[a], ctx->kf_alt[a]);
     }
     ctx->mixed = precision == SCFT_FP32X;
-    ctx->wsize = ctx->mixed ? 8 : 4;
     ctx->p.precision = precision;
     // the tiling carries over where the new form has it (its registers and LDS differ); else the
     // default variant and separate x and y passes
@@ -3855,7 +3851,7 @@ static int propagate_noalloc(scft_ctx &c, unsigned char *scr, int block, int sig
     if (e != hipSuccess) return set_err(nullptr, SCFT_E_HIP, std::string("table upload: ") + hipGetErrorString(e));
     size_t p = NA_TABLE_BYTES;
     c.wdz = scr + p;
-    p = align256(p + 4 * BN * c.wsize);
+    p = align256(p + 4 * BN * sizeof(T));
     C *sl[2] = {(C *)(scr + p), nullptr};
     p = align256(p + BN * sizeof(C));
     sl[1] = (C *)(scr + p);
@@ -3866,14 +3862,13 @@ static int propagate_noalloc(scft_ctx &c, unsigned char *scr, int block, int sig
     c.slots = sl[0];
     const int ns = block == 0 ? c.NsA : c.NsB;
     const long stride = (long)(ns + 1) * c.N;
-    if (c.wsize == 4)
-        hipLaunchKernelGGL(k_wdz_one<float>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
-                           (const double *)c.post_b, (float *)c.wdz + (size_t)block * BN,
-                           (float *)c.wdz + (size_t)(2 + block) * BN);
+    T *wpre = (T *)c.wdz + (size_t)block * BN, *wpost = (T *)c.wdz + (size_t)(2 + block) * BN;
+    if (c.mixed)
+        hipLaunchKernelGGL((k_wdz_one<T, true>), dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N,
+                           c.p.ds / 2, (const double *)c.post_b, wpre, wpost);
     else
-        hipLaunchKernelGGL(k_wdz_one<double>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N, c.p.ds / 2,
-                           (const double *)c.post_b, (double *)c.wdz + (size_t)block * BN,
-                           (double *)c.wdz + (size_t)(2 + block) * BN);
+        hipLaunchKernelGGL((k_wdz_one<T, false>), dim3(pt_grid(BN)), dim3(RT), 0, c.stream, w, (long)BN, c.N,
+                           c.p.ds / 2, (const double *)c.post_b, wpre, wpost);
     hipLaunchKernelGGL(k_seed_real<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0], q_init, (long)BN);
     hipLaunchKernelGGL(k_take_real_to<T>, dim3(pt_grid(BN)), dim3(RT), 0, c.stream, sl[0],
                        q_out + (sign == 1 ? 0 : (long)ns * c.N), (long)BN, c.N, stride);
@@ -3944,7 +3939,6 @@ extern "C" int scft_propagate_noalloc(const scft_params *pin, int device, int bl
     c.mixed = p.precision == SCFT_FP32X;
     c.csize = c.fp32 ? 8 : 16;
     c.tsize = c.fp32 ? 4 : 8;
-    c.wsize = (c.fp32 && !c.mixed) ? 4 : 8;
     for (int a = 0; a < 3; ++a) {
         c.n[a] = n[a];
         c.active[a] = n[a] > 1;
