@@ -44,7 +44,7 @@ struct PassArgs {
     using C = typename cplx<T>::type;
     const C *src;
     C *dst;
-    const void *W;       // the four W arrays in T (FP32X: the two pre arrays hold W - 1)
+    const T *W;          // the four W arrays (FP32X: the two pre arrays hold W - 1)
     unsigned wo[4];      // step == 0: element offsets of the four factors in W
     C *slot;             // step-boundary kernels: z(step) goes here, dst gets step + 1 opened
     unsigned wn[2];      // step-boundary kernels, step == 0: pre factors (real, imag) of step + 1

@@ -1637,7 +1637,7 @@ static PassArgs<T> pass_args(scft_ctx *c, int axis, const typename cplx<T>::type
     PassArgs<T> pa;
     pa.src = src;
     pa.dst = dst;
-    pa.W = c->wdz;  // [pre A, pre B, post A, post B]
+    pa.W = static_cast<const T *>(c->wdz);  // [pre A, pre B, post A, post B]
     pa.wo[0] = (unsigned)(br * BN);
     pa.wo[1] = (unsigned)(bi * BN);
     pa.wo[2] = (unsigned)((2 + br) * BN);
@@ -2477,7 +2477,7 @@ static hipError_t enqueue_pass_only(scft_ctx *c, int axis, int reps, int variant
     PassArgs<T> pa;
     pa.src = (const C *)c->work;
     pa.dst = (C *)c->work;
-    pa.W = c->wdz;
+    pa.W = static_cast<const T *>(c->wdz);
     pa.wo[0] = pa.wo[1] = pa.wo[2] = pa.wo[3] = 0u;
     // codes: see kPlaneCode.  A boundary kernel runs in place on the work
     // field, as in the march, with the arguments of consecutive contour steps.
