@@ -47,8 +47,10 @@ extern "C" {
 #define SCFT_FP64 0 /* everything FP64 (the reference's arithmetic) */
 #define SCFT_FP32 1 /* propagator, FFT and stored q in FP32; Q, phi, F, fields FP64 */
 #define SCFT_FP32X 2 /* stored q, W and every pass's loads and stores in FP32 (the HBM
-                        bytes of SCFT_FP32), the transform arithmetic in FP64 registers
-                        and LDS (rounded to FP32 once per store); Q, phi, F, fields FP64 */
+                        bytes of SCFT_FP32; the W factors applied before a step stored as
+                        W - 1, which FP32 keeps to about 1e-9), the transform arithmetic in
+                        FP64 registers and LDS (rounded to FP32 once per store); Q, phi,
+                        F, fields FP64 */
 
 /* field update */
 #define SCFT_MIX_SIMPLE 0   /* phi_w of the reference (cuda_aid.cu:382-416) */
@@ -216,8 +218,9 @@ int scft_autotune(scft_ctx *ctx, int reps, int *choice, float *ms);
 int scft_set_tiling(scft_ctx *ctx, const int *choice);
 
 /* Switch an FP32-storage context between SCFT_FP32 and SCFT_FP32X (no
-   reference counterpart).  Both forms keep the same FP32 slices, W factors and
-   FP64 fields, phi, Q, F and Anderson history, so a solve can run its bulk
+   reference counterpart).  Both forms keep the same FP32 slices and FP64
+   fields, phi, Q, F and Anderson history (each evaluation forms the W factors
+   in the current form's storage), so a solve can run its bulk
    iterations in SCFT_FP32 and finish in SCFT_FP32X, whose FP64 transform
    arithmetic removes the FP32 form's rounding floor.  The tiling carries over
    where the new form supports it (else the default); scft_autotune or
