@@ -91,6 +91,40 @@ def test_fp32x_full_size_single_eval(pkg, name):
     assert dF < 1e-7 and eQ < 2e-5 / np.sqrt(w["n"] ** 3) and dphi < 5e-7
 
 
+def test_fp32x_converged_gyroid_single_eval(pkg):
+    """One evaluation on converged 128^3 gyroid fields (400 FP64 Anderson
+    iterations from the seeded start, ridge 1e-2): FP32X against FP64.  On
+    these strongly segregated fields the FP32 rounding of W = exp(-w ds/2)
+    dominated FP32X's error while W was stored as such (relQ 1.07e-7, dF up
+    to 2e-7); stored as W - 1 it is relQ 8-9e-9, dF 3-4e-8, max|dphiA|
+    2-3e-8 (measured), the FP32 form relQ ~1e-6."""
+    name = "c3_gyroid_128"
+    print("case", name, "converged", flush=True)
+    w = pkg.fields.WORKLOADS[name]
+
+    def params(prec):
+        return _params(pkg, w["n"], w["L"], w["chiN"], w["fA"], w["ds"], prec, mixing=1,
+                       anderson_m=w["anderson_m"], anderson_mix=0.1, anderson_reg=1e-2)
+
+    with pkg.Solver(params(FP64)) as s:
+        s.set_fields(*pkg.fields.workload_fields(name))
+        s.iterate(400)
+        _, _, fA, fB = s.get_fields()
+    fA, fB = fA[0].copy(), fB[0].copy()
+    out = {}
+    for prec in (FP64, FP32X):
+        with pkg.Solver(params(prec)) as s:
+            s.set_fields(fA, fB)
+            Q = s.concentrations()[0]
+            phiA = s.get_fields()[0][0].copy()
+            s.update_fields()
+            out[prec] = (Q, s.free_energy()[0][0], phiA)
+    (q64, f64, a64), (q, f, a) = out[FP64], out[FP32X]
+    eQ, dF, dphi = abs(q - q64) / q64, abs(f - f64), float(np.max(np.abs(a - a64)))
+    print(f"  relQ={eQ:.3e} dF={dF:.3e} max|dphiA|={dphi:.3e}")
+    assert eQ < 2.5e-8 and dF < 1e-7 and dphi < 1e-7
+
+
 def test_fp32x_adjoint_identity_and_gradient(pkg):
     """The chain adjoint in FP32X: dlnQ/du = -phi/N, and the loss and gradient
     through 5 Anderson iterations against FP64 (the gradient is pointwise, so
